@@ -1,0 +1,270 @@
+// Status strings, error reporting, thread pool helper and the Matrix Market
+// reader/writer of libtaichi_amd.so (host code, built with g++).
+#include "common.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace tc {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+void parallel_for(long long n, const std::function<void(long long, long long)> &fn, int max_threads)
+{
+    int hw = (int)std::thread::hardware_concurrency();
+    if (hw <= 0) hw = 1;
+    int nt = max_threads > 0 ? max_threads : std::min(hw, 16);
+    if (n < 4096 || nt <= 1) {
+        if (n > 0) fn(0, n);
+        return;
+    }
+    nt = (int)std::min<long long>(nt, n);
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; t++) {
+        long long b = n * t / nt, e = n * (t + 1) / nt;
+        th.emplace_back([&fn, b, e] { fn(b, e); });
+    }
+    for (auto &t : th) t.join();
+}
+
+}  // namespace tc
+
+extern "C" {
+
+const char *tc_status_string(int s)
+{
+    switch (s) {
+    case TC_OK: return "ok";
+    case -1: return "cannot open file";
+    case -2: return "could not process Matrix Market banner";
+    case -3: return "complex matrices are not supported";
+    case -4: return "bad size line or data section";
+    case TC_ERR_INVALID: return "invalid argument";
+    case TC_ERR_HIP: return "HIP runtime error";
+    case TC_ERR_NOMEM: return "out of memory";
+    case TC_ERR_UNSUPPORTED: return "unsupported";
+    case TC_ERR_CAPACITY: return "over capacity";
+    case TC_ERR_NODEVICE: return "no HIP device";
+    case TC_ERR_STATE: return "wrong state";
+    default: return "unknown status";
+    }
+}
+
+const char *tc_last_error(void) { return tc::g_last_error.c_str(); }
+
+int tc_abi_version(void) { return TC_ABI_VERSION; }
+
+void tc_free_host(void *p) { free(p); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Matrix Market reader. Same results and return codes as readMtx
+// (TaiChi.cu:120-278) with mm_read_banner / mm_read_mtx_crd_size
+// (mmio.cpp:96-217): row order = file order, duplicates kept, pattern -> 1.0,
+// integer -> double -> float, symmetric/hermitian off-diagonals mirrored.
+// The data section is parsed from one buffer instead of fscanf per entry.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Cursor {
+    const char *p, *end;
+    void skip_ws()
+    {
+        while (p < end && isspace((unsigned char)*p)) ++p;
+    }
+    bool read_long(long long *v)
+    {
+        skip_ws();
+        if (p >= end) return false;
+        char *q;
+        errno = 0;
+        long long r = strtoll(p, &q, 10);
+        if (q == p) return false;
+        p = q;
+        *v = r;
+        return true;
+    }
+    bool read_double(double *v)
+    {
+        skip_ws();
+        if (p >= end) return false;
+        char *q;
+        double r = strtod(p, &q);
+        if (q == p) return false;
+        p = q;
+        *v = r;
+        return true;
+    }
+};
+
+// Banner fields as mm_read_banner classifies them. Returns 0 or an MM_* code.
+int parse_banner(const char *line, char tc[4])
+{
+    char banner[64], mtx[64], crd[64], dtype[64], storage[64];
+    tc[0] = tc[1] = tc[2] = ' ';
+    tc[3] = 'G';
+    if (sscanf(line, "%63s %63s %63s %63s %63s", banner, mtx, crd, dtype, storage) != 5) return 12;
+    auto lower = [](char *s) {
+        for (; *s; ++s) *s = (char)tolower((unsigned char)*s);
+    };
+    lower(mtx);
+    lower(crd);
+    lower(dtype);
+    lower(storage);
+    if (strncmp(banner, "%%MatrixMarket", 14) != 0) return 14;
+    if (strcmp(mtx, "matrix") != 0) return 15;
+    tc[0] = 'M';
+    if (!strcmp(crd, "coordinate")) tc[1] = 'C';
+    else if (!strcmp(crd, "array")) tc[1] = 'A';
+    else return 15;
+    if (!strcmp(dtype, "real")) tc[2] = 'R';
+    else if (!strcmp(dtype, "complex")) tc[2] = 'C';
+    else if (!strcmp(dtype, "pattern")) tc[2] = 'P';
+    else if (!strcmp(dtype, "integer")) tc[2] = 'I';
+    else return 15;
+    if (!strcmp(storage, "general")) tc[3] = 'G';
+    else if (!strcmp(storage, "symmetric")) tc[3] = 'S';
+    else if (!strcmp(storage, "hermitian")) tc[3] = 'H';
+    else if (!strcmp(storage, "skew-symmetric")) tc[3] = 'K';
+    else return 15;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA, int **rowptr,
+                           int **colidx, float **val)
+{
+    if (!path || !m || !n || !nnzA || !rowptr || !colidx || !val)
+        return tc::fail(TC_ERR_INVALID, "tc_read_mtx: null argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return tc::fail(-1, "cannot open %s", path);
+    std::vector<char> buf;
+    {
+        char tmp[1 << 16];
+        size_t got;
+        while ((got = fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+        fclose(f);
+    }
+    buf.push_back('\0');
+    const char *p = buf.data(), *end = buf.data() + buf.size() - 1;
+    // banner: first line (fgets of at most 1024 chars, mmio.cpp:109)
+    const char *nl = (const char *)memchr(p, '\n', end - p);
+    std::string first(p, nl ? nl : end);
+    if (first.size() > 1024) first.resize(1024);
+    char tc[4];
+    if (first.empty() && !nl) return tc::fail(-2, "empty file");
+    if (parse_banner(first.c_str(), tc) != 0) return tc::fail(-2, "bad Matrix Market banner");
+    if (tc[2] == 'C') return tc::fail(-3, "complex matrices are not supported");
+    p = nl ? nl + 1 : end;
+    // size line: skip lines starting with '%' (mmio.cpp:199-203)
+    for (;;) {
+        if (p >= end) return tc::fail(-4, "premature end before the size line");
+        const char *e = (const char *)memchr(p, '\n', end - p);
+        if (*p != '%') break;
+        p = e ? e + 1 : end;
+    }
+    Cursor cur{p, end};
+    long long M, N, NZ;
+    if (!cur.read_long(&M) || !cur.read_long(&N) || !cur.read_long(&NZ))
+        return tc::fail(-4, "bad size line");
+    // 'array' files: the reference misreads their two-number size line; rejected.
+    if (tc[1] != 'C' || M < 0 || N < 0 || NZ < 0 || M > INT32_MAX || N > INT32_MAX)
+        return tc::fail(-4, "unsupported size line or storage");
+    const bool isReal = tc[2] == 'R', isInt = tc[2] == 'I';
+    const bool isSym = tc[3] == 'S' || tc[3] == 'H';
+    std::vector<int> ri((size_t)NZ), ci((size_t)NZ);
+    std::vector<float> vi((size_t)NZ);
+    std::vector<long long> cnt((size_t)M + 1, 0);
+    for (long long i = 0; i < NZ; i++) {
+        long long a, b;
+        double fv = 1.0;
+        if (!cur.read_long(&a) || !cur.read_long(&b))
+            return tc::fail(-4, "data section ends at entry %lld", i);
+        if (isReal) {
+            if (!cur.read_double(&fv)) return tc::fail(-4, "missing value at entry %lld", i);
+        } else if (isInt) {
+            long long iv;
+            if (!cur.read_long(&iv)) return tc::fail(-4, "missing value at entry %lld", i);
+            fv = (double)(int)iv;
+        }
+        a--;
+        b--;
+        if (a < 0 || a >= M || b < 0 || b >= N)
+            return tc::fail(-4, "entry %lld out of range", i);
+        ri[i] = (int)a;
+        ci[i] = (int)b;
+        vi[i] = (float)fv;
+        cnt[a]++;
+        if (isSym && a != b) cnt[b]++;
+    }
+    long long total = 0;
+    for (long long r = 0; r < M; r++) total += cnt[r];
+    if (total > INT32_MAX) return tc::fail(-4, "too many entries");
+    int *rp = (int *)malloc(((size_t)M + 1) * sizeof(int));
+    int *cc = (int *)malloc((size_t)(total ? total : 1) * sizeof(int));
+    float *vv = (float *)malloc((size_t)(total ? total : 1) * sizeof(float));
+    if (!rp || !cc || !vv) {
+        free(rp);
+        free(cc);
+        free(vv);
+        return tc::fail(TC_ERR_NOMEM, "tc_read_mtx: out of host memory");
+    }
+    rp[0] = 0;
+    for (long long r = 0; r < M; r++) rp[r + 1] = rp[r] + (int)cnt[r];
+    std::vector<int> fillp(rp, rp + M);
+    for (long long i = 0; i < NZ; i++) {
+        int a = ri[i], b = ci[i];
+        int o = fillp[a]++;
+        cc[o] = b;
+        vv[o] = vi[i];
+        if (isSym && a != b) {
+            o = fillp[b]++;
+            cc[o] = a;
+            vv[o] = vi[i];
+        }
+    }
+    *m = (int)M;
+    *n = (int)N;
+    *nnzA = (unsigned long)total;
+    *rowptr = rp;
+    *colidx = cc;
+    *val = vv;
+    return 0;
+}
+
+extern "C" int tc_write_mtx(const char *path, int M, int N, const int *rowptr, const int *col)
+{
+    if (!path || M < 0 || N < 0 || !rowptr || (rowptr[M] > 0 && !col))
+        return tc::fail(TC_ERR_INVALID, "tc_write_mtx: bad argument");
+    FILE *f = fopen(path, "w");
+    if (!f) return tc::fail(-1, "cannot create %s", path);
+    fprintf(f, "%%%%MatrixMarket matrix coordinate pattern general\n%d %d %d\n", M, N, rowptr[M]);
+    for (int r = 0; r < M; r++)
+        for (int j = rowptr[r]; j < rowptr[r + 1]; j++) fprintf(f, "%d %d\n", r + 1, col[j] + 1);
+    int bad = ferror(f);
+    fclose(f);
+    return bad ? tc::fail(-1, "write error on %s", path) : TC_OK;
+}

@@ -1,0 +1,26 @@
+// Internal interface between the partition (partition.cpp), the thumbnail
+// morphology (morph.cpp) and the reference-compatible layer (compat.cpp).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace tc {
+
+// getSdf's image pipeline: x1,y1,x2,y2 per kept contour
+std::vector<int> sdf_from_image(const uint8_t *mat, int H, int W);
+// contours alone: [n, (len, hole, x0, y0, ...)...]
+std::vector<int> contours_flat(const uint8_t *img, int H, int W);
+
+// One square shape's diagonal partition (partition_dia_shape semantics).
+struct DiaShape {
+    int xs = 0, ys = 0, side = 0, width = 0;
+    std::vector<int> offs;                 // dense diagonal offsets, increasing
+    std::vector<int> st, en;               // row range [st, en) of each dense diagonal
+    std::vector<std::vector<int>> zeros;   // per dense diagonal: zero rows (local)
+    long long nnz = 0;                     // claimed nonzeros
+};
+void partition_square(const int *rowptr, const int *col, std::vector<uint8_t> &claimed, int x, int y, int side,
+                      DiaShape &sh);
+
+}  // namespace tc

@@ -1,0 +1,947 @@
+// Hybrid diagonal + CSR SpMV for gfx950 (kernels (b) and (c), fused).
+//
+// Replaces taichi_SpMV's upload and CSR_coop_spmv_kernel{,_hybrid1,2,3}
+// (TaiChi.cu:965-1456). y = A x with A in the TaiChi_new format: dense
+// diagonal segments of square shapes plus a value-free, sign-encoded CSR
+// remainder (csrCol = col<<1 | sign; sign 0 subtracts x, compensating a zero
+// inside a dense diagonal).
+//
+// Device layout (built by tc_matrix_create from a host TaiChi_new):
+//   shapes[S]   {xStart, xStop, yStart, off_begin, off_end, off_min, off_max}
+//   offs[D]     dense diagonal offsets
+//   gbase[G+1]  start of each 64-row group's short-row entries in cols
+//   roff[M]     u8 or u16: end of row i's short entries, relative to gbase[i/64]
+//   cols[R]     encoded entries: short rows in row order, then long rows
+//   glong[G+1], lrow[L], lptr[L+1]   rows longer than the long-row threshold
+// Compared with the reference's int rowptr (4 bytes per row) the per-row
+// cost is 1 (or 2) bytes plus 4 bytes per 64 rows.
+//
+// One workgroup (4 wavefronts) computes a tile of TRW = 256*P/K rows for K
+// right-hand sides:
+//   1. row extents of the tile into LDS;
+//   2. short-row entries of the tile streamed with coalesced loads, x gathered
+//      and sign applied, values staged in LDS (chunks of `capv` floats); each
+//      (row, column) pair is then summed serially by one lane in entry order;
+//   3. each long row is summed by one wavefront (64-lane strided partial sums,
+//      DPP/ds_swizzle butterfly, fixed order);
+//   4. every diagonal shape overlapping the tile adds x[r - xS + yS + o] for its
+//      offsets o inside the square's column window; the x window of the tile
+//      is staged once in LDS when a shape has two or more offsets, so no
+//      column index is read for the dense part;
+//   5. y stored once, coalesced.
+// The per-(row, column) summation order depends only on the format (short
+// rows serial, long rows a fixed butterfly, then shapes and offsets in stored
+// order), never on the tile size, the grid or the batch a request runs in, so
+// a request gives bit-identical y alone, in a batch, or with another config.
+//
+// No MFMA: one add per stored index, bound by HBM bandwidth.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+constexpr int kWG = 256;
+constexpr int kWaves = kWG / 64;
+
+struct alignas(16) ShapeDev {
+    int xs, xe, ys, ob, oe, omin, omax, pad;
+};
+
+// Device pointers and sizes of one resident matrix; passed by value.
+struct MatView {
+    int M, N, S, L;
+    int ow;        // bytes per local row offset: 1 or 2
+    int ngroups;
+    const ShapeDev *shapes;
+    const int *offs;
+    const int *gbase;
+    const void *roff;
+    const int *cols;
+    const int *glong;
+    const int *lrow;
+    const int *lptr;
+};
+
+struct bf16_t {
+    uint16_t bits;
+};
+typedef _Float16 f16_t;
+
+template <typename XT>
+__device__ __forceinline__ float load_x(const XT *__restrict__ X, size_t i);
+template <>
+__device__ __forceinline__ float load_x<float>(const float *__restrict__ X, size_t i)
+{
+    return X[i];
+}
+template <>
+__device__ __forceinline__ float load_x<bf16_t>(const bf16_t *__restrict__ X, size_t i)
+{
+    return __uint_as_float((uint32_t)X[i].bits << 16);
+}
+template <>
+__device__ __forceinline__ float load_x<f16_t>(const f16_t *__restrict__ X, size_t i)
+{
+    return (float)X[i];
+}
+
+// start of row i's short entries (i in [0, M]); row i ends at row_start(i+1)
+__device__ __forceinline__ int row_start(const MatView &A, int i)
+{
+    int b = A.gbase[i >> 6];
+    if (i & 63) b += A.ow == 1 ? (int)((const uint8_t *)A.roff)[i - 1] : (int)((const uint16_t *)A.roff)[i - 1];
+    return b;
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed order: quad swaps and
+// row mirrors by DPP, the 16-lane swap by ds_swizzle, halves by readlane.
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
+    v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));                      // lane ^ 16
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+}
+
+// Bijection of block ids so that each XCD (blocks are dispatched round-robin
+// over the 8 XCDs) gets a contiguous range of slots, hence contiguous rows
+// and x windows in its own L2.
+__device__ __forceinline__ int xcd_slot(int b, int G)
+{
+    const int q = G >> 3, r = G & 7, x = b & 7, l = b >> 3;
+    return x * q + min(x, r) + l;
+}
+
+template <int K, typename XT, int P>
+__device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
+                                         int ldy, int u0, float *s_val, int capv, int *s_start, float *s_lsum)
+{
+    constexpr int TRW = (kWG * P) / K;
+    constexpr int NP = TRW * K;
+    const int tid = threadIdx.x;
+    const int rows = min(TRW, A.M - u0);
+
+    // 1. row extents
+    for (int r = tid; r <= TRW; r += kWG) s_start[r] = row_start(A, min(u0 + r, A.M));
+    __syncthreads();
+    const int ebase = s_start[0];
+    const int E = s_start[TRW] - ebase;
+
+    float acc[P];
+    int prow[P], pcol[P];
+    bool pv[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int q = tid + p * kWG;
+        prow[p] = q / K;
+        pcol[p] = q - prow[p] * K;
+        pv[p] = q < NP && prow[p] < rows;
+        acc[p] = 0.0f;
+    }
+
+    // 2. short rows: stream entries, stage signed x values, serial row sums
+    const int cape = capv / K;
+    for (int e0 = 0; e0 < E; e0 += cape) {
+        const int ne = min(cape, E - e0);
+        const int *__restrict__ cp = A.cols + ebase + e0;
+        for (int idx = tid; idx < ne * K; idx += kWG) {
+            const int j = idx / K, c = idx - j * K;
+            const int v = cp[j];
+            const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
+            s_val[idx] = (v & 1) ? xv : -xv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            if (!pv[p]) continue;
+            const int lo = max(s_start[prow[p]] - ebase, e0);
+            const int hi = min(s_start[prow[p] + 1] - ebase, e0 + ne);
+            float a = acc[p];
+            for (int j = lo; j < hi; j++) a += s_val[(j - e0) * K + pcol[p]];
+            acc[p] = a;
+        }
+        __syncthreads();
+    }
+
+    // 3. long rows: one wavefront per row
+    if (A.L > 0) {
+        const int lb = A.glong[u0 >> 6], le = A.glong[((u0 + rows - 1) >> 6) + 1];
+        if (lb < le) {
+            for (int q = tid; q < NP; q += kWG) s_lsum[q] = 0.0f;
+            __syncthreads();
+            const int wave = tid >> 6, lane = tid & 63;
+            for (int l = lb + wave; l < le; l += kWaves) {
+                const int row = A.lrow[l];
+                if (row < u0 || row >= u0 + rows) continue;
+                const int j0 = A.lptr[l], j1 = A.lptr[l + 1];
+                for (int c = 0; c < K; c++) {
+                    float part = 0.0f;
+                    for (int j = j0 + lane; j < j1; j += 64) {
+                        const int v = A.cols[j];
+                        const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
+                        part += (v & 1) ? xv : -xv;
+                    }
+                    part = wave_sum(part);
+                    if (lane == 0) s_lsum[(row - u0) * K + c] = part;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < P; p++)
+                if (pv[p]) acc[p] += s_lsum[tid + p * kWG];
+            __syncthreads();
+        }
+    }
+
+    // 4. diagonal segments
+    for (int s = 0; s < A.S; s++) {
+        const ShapeDev sh = A.shapes[s];
+        const int ra = max(u0, sh.xs), rb = min(u0 + rows, sh.xe);
+        if (ra >= rb) continue;
+        const int shift = sh.ys - sh.xs;
+        const int clo = sh.ys, chi = sh.ys + (sh.xe - sh.xs);
+        const int w0 = max(clo, ra + shift + sh.omin);
+        const int w1 = min(chi, rb - 1 + shift + sh.omax + 1);
+        const bool stage = (sh.oe - sh.ob) >= 2 && w1 > w0 && (w1 - w0) * K <= capv;
+        if (stage) {
+            for (int idx = tid; idx < (w1 - w0) * K; idx += kWG) {
+                const int cc = idx / K, c = idx - cc * K;
+                s_val[idx] = load_x<XT>(X, (size_t)(w0 + cc) * ldx + c);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const int r = u0 + prow[p];
+            if (!pv[p] || r < ra || r >= rb) continue;
+            const int base = r + shift;
+            float a = acc[p];
+            for (int n = sh.ob; n < sh.oe; n++) {
+                const int c = base + A.offs[n];
+                if (c >= clo && c < chi)
+                    a += stage ? s_val[(c - w0) * K + pcol[p]] : load_x<XT>(X, (size_t)c * ldx + pcol[p]);
+            }
+            acc[p] = a;
+        }
+        if (stage) __syncthreads();
+    }
+
+    // 5. store
+#pragma unroll
+    for (int p = 0; p < P; p++)
+        if (pv[p]) Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]] = acc[p];
+}
+
+// LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
+__host__ __device__ constexpr size_t lds_bytes(int capv, int P, int trw)
+{
+    return ((size_t)capv + (size_t)kWG * P + (size_t)trw + 1) * 4;
+}
+
+template <int K, typename XT, int P>
+__global__ __launch_bounds__(kWG) void spmv_kernel(MatView A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
+                                                   int ldy, int nunits, int capv)
+{
+    extern __shared__ float smem[];
+    constexpr int TRW = (kWG * P) / K;
+    float *s_val = smem;
+    float *s_lsum = s_val + capv;
+    int *s_start = (int *)(s_lsum + kWG * P);
+    const int G = gridDim.x;
+    const int v = xcd_slot(blockIdx.x, G);
+    const int ub = (int)((long long)v * nunits / G), ue = (int)((long long)(v + 1) * nunits / G);
+    for (int u = ub; u < ue; u++) {
+        run_tile<K, XT, P>(A, X, ldx, Y, ldy, u * TRW, s_val, capv, s_start, s_lsum);
+        __syncthreads();
+    }
+}
+
+// ---- batched persistent kernel ----
+constexpr int kBatchP = 2;
+
+struct WorkDesc {
+    MatView A;
+    const void *X;
+    float *Y;
+    int ldx, ldy, K, dtype;
+    int unit_begin, pad0;
+};
+
+template <int K, typename XT>
+__device__ __forceinline__ void batch_unit(const WorkDesc &w, int local, float *s_val, int capv, int *s_start,
+                                           float *s_lsum)
+{
+    constexpr int TRW = (kWG * kBatchP) / K;
+    run_tile<K, XT, kBatchP>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, local * TRW, s_val, capv, s_start, s_lsum);
+}
+
+template <typename XT>
+__device__ __forceinline__ void batch_dispatch_k(const WorkDesc &w, int local, float *s_val, int capv, int *s_start,
+                                                 float *s_lsum)
+{
+    switch (w.K) {
+    case 1: batch_unit<1, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    case 2: batch_unit<2, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    case 3: batch_unit<3, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    case 4: batch_unit<4, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    case 5: batch_unit<5, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    case 6: batch_unit<6, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    case 7: batch_unit<7, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    default: batch_unit<8, XT>(w, local, s_val, capv, s_start, s_lsum); break;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void batch_kernel(const WorkDesc *__restrict__ D, int ndesc, int total_units,
+                                                    int capv)
+{
+    extern __shared__ float smem[];
+    float *s_val = smem;
+    float *s_lsum = s_val + capv;
+    int *s_start = (int *)(s_lsum + kWG * kBatchP);
+    const int G = gridDim.x;
+    const int v = xcd_slot(blockIdx.x, G);
+    const int ub = (int)((long long)v * total_units / G), ue = (int)((long long)(v + 1) * total_units / G);
+    if (ub >= ue) return;
+    int lo = 0, hi = ndesc - 1;  // last descriptor with unit_begin <= ub
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (D[mid].unit_begin <= ub) lo = mid;
+        else hi = mid - 1;
+    }
+    int d = lo;
+    for (int u = ub; u < ue; u++) {
+        while (d + 1 < ndesc && D[d + 1].unit_begin <= u) d++;
+        const WorkDesc w = D[d];
+        const int local = u - w.unit_begin;
+        if (w.dtype == TC_BF16) batch_dispatch_k<bf16_t>(w, local, s_val, capv, s_start, s_lsum);
+        else if (w.dtype == TC_F16) batch_dispatch_k<f16_t>(w, local, s_val, capv, s_start, s_lsum);
+        else batch_dispatch_k<float>(w, local, s_val, capv, s_start, s_lsum);
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+#define HIPCHECK(expr)                                                                                      \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) return tc::fail(TC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+struct tc_matrix {
+    int device = 0;
+    MatView v{};
+    void *mem = nullptr;
+    size_t mem_bytes = 0;
+    tc_matrix_stats st{};
+    std::mutex mu;
+    std::map<std::pair<int, int>, tc_config> cfg;
+};
+
+struct tc_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int g_cu_count = 0;
+
+int cu_count()
+{
+    if (g_cu_count == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            g_cu_count = prop.multiProcessorCount;
+        else
+            g_cu_count = 256;
+    }
+    return g_cu_count;
+}
+
+int pow2_floor(int x)
+{
+    int p = 1;
+    while (p * 2 <= x) p *= 2;
+    return p;
+}
+
+// supported compile-time column counts, largest first
+const int kKs[] = {64, 32, 16, 8, 7, 6, 5, 4, 3, 2, 1};
+
+int pick_k(int k)
+{
+    for (int K : kKs)
+        if (K <= k) return K;
+    return 1;
+}
+
+tc_config default_config(int K)
+{
+    tc_config c{};
+    c.rows_per_tile = (kWG * 2) / K;  // P = 2
+    c.lds_values = 4096;
+    c.grid = 0;
+    return c;
+}
+
+// P from a rows_per_tile request for column count K (1, 2 or 4)
+int p_for(int K, int rows_per_tile)
+{
+    int best = 2;
+    for (int P : {1, 2, 4})
+        if ((kWG * P) / K == rows_per_tile) best = P;
+    return best;
+}
+
+template <int K, typename XT, int P>
+int launch_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int capv, int grid, hipStream_t s)
+{
+    constexpr int TRW = (kWG * P) / K;
+    const int nunits = (A.M + TRW - 1) / TRW;
+    if (nunits == 0) return TC_OK;
+    int G = grid > 0 ? std::min(grid, nunits) : nunits;
+    const size_t lds = lds_bytes(capv, P, TRW);
+    spmv_kernel<K, XT, P><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, nunits, capv);
+    HIPCHECK(hipGetLastError());
+    return TC_OK;
+}
+
+template <int K, typename XT>
+int launch_p(const MatView &A, const void *X, int ldx, float *Y, int ldy, int P, int capv, int grid, hipStream_t s)
+{
+    if (P == 1) return launch_t<K, XT, 1>(A, X, ldx, Y, ldy, capv, grid, s);
+    if (P == 4) return launch_t<K, XT, 4>(A, X, ldx, Y, ldy, capv, grid, s);
+    return launch_t<K, XT, 2>(A, X, ldx, Y, ldy, capv, grid, s);
+}
+
+template <typename XT>
+int launch_k(const MatView &A, const void *X, int ldx, float *Y, int ldy, int K, int P, int capv, int grid,
+             hipStream_t s)
+{
+    switch (K) {
+    case 1: return launch_p<1, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 2: return launch_p<2, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 3: return launch_p<3, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 4: return launch_p<4, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 5: return launch_p<5, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 6: return launch_p<6, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 7: return launch_p<7, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 8: return launch_p<8, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 16: return launch_p<16, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    case 32: return launch_p<32, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    default: return launch_p<64, XT>(A, X, ldx, Y, ldy, P, capv, grid, s);
+    }
+}
+
+int elem_size(int dtype) { return dtype == TC_F32 ? 4 : 2; }
+
+tc_config get_cfg(tc_matrix *A, int K, int dtype)
+{
+    std::lock_guard<std::mutex> g(A->mu);
+    auto it = A->cfg.find({K, dtype});
+    return it != A->cfg.end() ? it->second : default_config(K);
+}
+
+// one launch of K columns (K a supported count) with an explicit config
+int launch_cfg(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int K, const tc_config &c,
+               hipStream_t s)
+{
+    const int P = p_for(K, c.rows_per_tile);
+    const int capv = std::max(c.lds_values, 64 * K);
+    if (dtype == TC_BF16) return launch_k<bf16_t>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
+    if (dtype == TC_F16) return launch_k<f16_t>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
+    return launch_k<float>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
+}
+
+int check_spmv_args(const tc_matrix *A, const void *X, int ldx, int dtype, const float *Y, int ldy, int k)
+{
+    if (!A) return tc::fail(TC_ERR_INVALID, "tc_spmv: null matrix");
+    if (dtype != TC_F32 && dtype != TC_BF16 && dtype != TC_F16) return tc::fail(TC_ERR_INVALID, "bad dtype %d", dtype);
+    if (k < 1 || ldx < k || ldy < k) return tc::fail(TC_ERR_INVALID, "bad k=%d ldx=%d ldy=%d", k, ldx, ldy);
+    if ((A->v.N > 0 && !X) || (A->v.M > 0 && !Y)) return tc::fail(TC_ERR_INVALID, "null X or Y");
+    return TC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tc_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int tc_matrix_create(int M, int N, const TaiChi_new *T, long long nnz, int long_threshold, void *stream,
+                     tc_matrix **out)
+{
+    if (!T || !out || M < 0 || N < 0) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: bad argument");
+    *out = nullptr;
+    if (M > 0 && !T->csrRow) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: null csrRow");
+    const int LT = long_threshold > 0 ? std::min(long_threshold, 1023) : tc::kDefaultLongRow;
+    const int S = T->dia_shapes;
+    const int R = M > 0 ? T->csrRow[M] : 0;
+    if (S < 0 || R < 0 || (R > 0 && !T->csrCol)) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: bad format");
+    if (S > 0 && (!T->xStart || !T->xStop || !T->yStart || !T->nDiasPtr || (T->nDiasPtr[S] > 0 && !T->neg_offsets)))
+        return tc::fail(TC_ERR_INVALID, "tc_matrix_create: null shape arrays");
+    const bool encoded = T->dense_nnz > 0;
+    if (!encoded && N >= (1 << 30)) return tc::fail(TC_ERR_UNSUPPORTED, "tc_matrix_create: N >= 2^30");
+    const int D = S > 0 ? T->nDiasPtr[S] : 0;
+
+    // shapes: validated so that every column the kernel forms lies in [0, N)
+    std::vector<ShapeDev> shapes((size_t)S);
+    for (int s = 0; s < S; s++) {
+        ShapeDev &h = shapes[s];
+        h.xs = T->xStart[s];
+        h.xe = T->xStop[s];
+        h.ys = T->yStart[s];
+        h.ob = T->nDiasPtr[s];
+        h.oe = T->nDiasPtr[s + 1];
+        if (h.xs < 0 || h.xe > M || h.xs > h.xe || h.ys < 0 || (long long)h.ys + (h.xe - h.xs) > N || h.ob < 0 ||
+            h.oe < h.ob || h.oe > D)
+            return tc::fail(TC_ERR_INVALID, "tc_matrix_create: shape %d out of range", s);
+        h.omin = 0;
+        h.omax = 0;
+        if (h.oe > h.ob) {
+            h.omin = h.omax = T->neg_offsets[h.ob];
+            for (int n = h.ob; n < h.oe; n++) {
+                h.omin = std::min(h.omin, T->neg_offsets[n]);
+                h.omax = std::max(h.omax, T->neg_offsets[n]);
+            }
+        }
+        h.pad = 0;
+    }
+    // rows: lengths, long rows, group offsets
+    const int G = (M + 63) / 64;
+    std::vector<int> gbase((size_t)G + 1, 0), lrow, lptr;
+    std::vector<uint16_t> roff16((size_t)M);
+    long long short_total = 0;
+    int maxoff = 0;
+    for (int r = 0; r < M; r++) {
+        const int b = T->csrRow[r], e = T->csrRow[r + 1];
+        if (e < b) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: csrRow not monotone at %d", r);
+        if ((r & 63) == 0) gbase[r >> 6] = (int)short_total;
+        if (e - b > LT) lrow.push_back(r);
+        else short_total += e - b;
+        const int o = (int)(short_total - gbase[r >> 6]);
+        roff16[r] = (uint16_t)o;
+        maxoff = std::max(maxoff, o);
+    }
+    gbase[G] = (int)short_total;
+    const int L = (int)lrow.size();
+    const int ow = maxoff <= 255 ? 1 : 2;
+    std::vector<int> glong;
+    if (L > 0) {
+        glong.assign((size_t)G + 1, 0);
+        for (int r : lrow) glong[(r >> 6) + 1]++;
+        for (int g = 0; g < G; g++) glong[g + 1] += glong[g];
+        lptr.resize((size_t)L + 1);
+        lptr[0] = (int)short_total;
+        for (int l = 0; l < L; l++) lptr[l + 1] = lptr[l] + (T->csrRow[lrow[l] + 1] - T->csrRow[lrow[l]]);
+    }
+    // encoded cols: short rows in row order, then long rows; checked against N
+    std::vector<int> cols((size_t)R);
+    {
+        std::vector<long long> dst((size_t)M);
+        long long sp = 0, lp = short_total;
+        size_t li = 0;
+        for (int r = 0; r < M; r++) {
+            const int len = T->csrRow[r + 1] - T->csrRow[r];
+            if (li < lrow.size() && lrow[li] == r) {
+                dst[r] = lp;
+                lp += len;
+                li++;
+            } else {
+                dst[r] = sp;
+                sp += len;
+            }
+        }
+        int bad = 0;
+        std::mutex bm;
+        tc::parallel_for(M, [&](long long b, long long e) {
+            int lbad = 0;
+            for (long long r = b; r < e; r++) {
+                long long o = dst[r];
+                for (int j = T->csrRow[r]; j < T->csrRow[r + 1]; j++) {
+                    const int raw = T->csrCol[j];
+                    const int c = encoded ? raw >> 1 : raw;
+                    if (raw < 0 || c >= N) {
+                        lbad = 1;
+                        cols[o++] = 1;
+                        continue;
+                    }
+                    cols[o++] = encoded ? raw : ((raw << 1) | 1);
+                }
+            }
+            if (lbad) {
+                std::lock_guard<std::mutex> g(bm);
+                bad = 1;
+            }
+        });
+        if (bad) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: column index out of range");
+    }
+
+    // one device allocation, 256-byte aligned sub-arrays
+    struct Part {
+        const void *src;
+        size_t bytes;
+        size_t off;
+    };
+    std::vector<uint8_t> roff8;
+    if (ow == 1) {
+        roff8.resize((size_t)M);
+        for (int r = 0; r < M; r++) roff8[r] = (uint8_t)roff16[r];
+    }
+    Part parts[] = {
+        {shapes.data(), shapes.size() * sizeof(ShapeDev), 0},
+        {S > 0 ? T->neg_offsets : nullptr, (size_t)D * sizeof(int), 0},
+        {gbase.data(), gbase.size() * sizeof(int), 0},
+        {ow == 1 ? (const void *)roff8.data() : (const void *)roff16.data(), (size_t)M * ow, 0},
+        {cols.data(), cols.size() * sizeof(int), 0},
+        {glong.data(), glong.size() * sizeof(int), 0},
+        {lrow.data(), lrow.size() * sizeof(int), 0},
+        {lptr.data(), lptr.size() * sizeof(int), 0},
+    };
+    size_t total = 0, fmt_bytes = 0;
+    for (Part &p : parts) {
+        p.off = total;
+        total += (p.bytes + 255) & ~(size_t)255;
+        fmt_bytes += p.bytes;
+    }
+    tc_matrix *A = new tc_matrix();
+    if (hipGetDevice(&A->device) != hipSuccess) {
+        delete A;
+        return tc::fail(TC_ERR_NODEVICE, "tc_matrix_create: no HIP device");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    if (total > 0) {
+        hipError_t e = hipMalloc(&A->mem, total);
+        if (e != hipSuccess) {
+            delete A;
+            return tc::fail(TC_ERR_NOMEM, "tc_matrix_create: hipMalloc(%zu): %s", total, hipGetErrorString(e));
+        }
+        for (Part &p : parts)
+            if (p.bytes) {
+                e = hipMemcpyAsync((char *)A->mem + p.off, p.src, p.bytes, hipMemcpyHostToDevice, s);
+                if (e != hipSuccess) {
+                    (void)hipFree(A->mem);
+                    delete A;
+                    return tc::fail(TC_ERR_HIP, "tc_matrix_create: upload: %s", hipGetErrorString(e));
+                }
+            }
+        hipError_t se = hipStreamSynchronize(s);
+        if (se != hipSuccess) {
+            (void)hipFree(A->mem);
+            delete A;
+            return tc::fail(TC_ERR_HIP, "tc_matrix_create: sync: %s", hipGetErrorString(se));
+        }
+    }
+    auto at = [&](int i) -> const void * { return parts[i].bytes ? (const char *)A->mem + parts[i].off : nullptr; };
+    A->mem_bytes = total;
+    MatView &v = A->v;
+    v.M = M;
+    v.N = N;
+    v.S = S;
+    v.L = L;
+    v.ow = ow;
+    v.ngroups = G;
+    v.shapes = (const ShapeDev *)at(0);
+    v.offs = (const int *)at(1);
+    v.gbase = (const int *)at(2);
+    v.roff = at(3);
+    v.cols = (const int *)at(4);
+    v.glong = (const int *)at(5);
+    v.lrow = (const int *)at(6);
+    v.lptr = (const int *)at(7);
+    tc_matrix_stats &st = A->st;
+    st.M = M;
+    st.N = N;
+    st.dia_shapes = S;
+    st.num_negs = D;
+    st.long_rows = L;
+    st.offset_bytes = ow;
+    st.nnz = nnz;
+    st.remainder = R;
+    st.format_bytes = (long long)fmt_bytes;
+    st.device_bytes = (long long)total;
+    *out = A;
+    return TC_OK;
+}
+
+int tc_matrix_destroy(tc_matrix *A)
+{
+    if (!A) return TC_OK;
+    if (A->mem) (void)hipFree(A->mem);
+    delete A;
+    return TC_OK;
+}
+
+int tc_matrix_get_stats(const tc_matrix *A, tc_matrix_stats *out)
+{
+    if (!A || !out) return tc::fail(TC_ERR_INVALID, "tc_matrix_get_stats: null argument");
+    *out = A->st;
+    return TC_OK;
+}
+
+long long tc_spmv_bytes(const tc_matrix *A, int k, int dtype)
+{
+    if (!A || k < 1) return -1;
+    return A->st.format_bytes + (long long)elem_size(dtype) * A->st.N * k + 4ll * A->st.M * k;
+}
+
+int tc_spmv(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int k, void *stream)
+{
+    int rc = check_spmv_args(A, X, ldx, dtype, Y, ldy, k);
+    if (rc != TC_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t es = elem_size(dtype);
+    // columns in chunks of supported widths; each column's result does not
+    // depend on the chunking
+    for (int c0 = 0; c0 < k;) {
+        const int K = pick_k(k - c0);
+        const tc_config c = get_cfg(A, K, dtype);
+        rc = launch_cfg(A, (const char *)X + c0 * es, ldx, dtype, Y + c0, ldy, K, c, s);
+        if (rc != TC_OK) return rc;
+        c0 += K;
+    }
+    return TC_OK;
+}
+
+int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg)
+{
+    if (!A || !cfg || k < 1) return tc::fail(TC_ERR_INVALID, "tc_matrix_set_config: bad argument");
+    const int K = pick_k(k);
+    tc_config c = *cfg;
+    const tc_config d = default_config(K);
+    if (c.rows_per_tile <= 0) c.rows_per_tile = d.rows_per_tile;
+    if (c.lds_values <= 0) c.lds_values = d.lds_values;
+    bool ok = false;
+    for (int P : {1, 2, 4})
+        if ((kWG * P) / K == c.rows_per_tile) ok = true;
+    if (!ok) return tc::fail(TC_ERR_INVALID, "rows_per_tile %d not available for k=%d", c.rows_per_tile, K);
+    if (c.lds_values < 512 || c.lds_values > 8192)
+        return tc::fail(TC_ERR_INVALID, "lds_values %d outside [512, 8192]", c.lds_values);
+    if (c.grid < 0) return tc::fail(TC_ERR_INVALID, "grid < 0");
+    std::lock_guard<std::mutex> g(A->mu);
+    A->cfg[{K, dtype}] = c;
+    return TC_OK;
+}
+
+int tc_matrix_get_config(const tc_matrix *A, int k, int dtype, tc_config *cfg)
+{
+    if (!A || !cfg || k < 1) return tc::fail(TC_ERR_INVALID, "tc_matrix_get_config: bad argument");
+    *cfg = get_cfg(const_cast<tc_matrix *>(A), pick_k(k), dtype);
+    return TC_OK;
+}
+
+int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int k, void *stream,
+                tc_config *best, float *best_us)
+{
+    int rc = check_spmv_args(A, X, ldx, dtype, Y, ldy, k);
+    if (rc != TC_OK) return rc;
+    const int K = pick_k(k);
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    const int ncu = cu_count();
+    std::vector<tc_config> cands;
+    for (int P : {1, 2, 4})
+        for (int capv : {2048, 4096, 8192})
+            for (int grid : {0, 4 * ncu, 8 * ncu}) {
+                if (capv < 64 * K) continue;
+                tc_config c{(kWG * P) / K, capv, grid};
+                cands.push_back(c);
+            }
+    tc_config bc = default_config(K);
+    float bt = 1e30f;
+    const int reps = 20;
+    for (const tc_config &c : cands) {
+        for (int w = 0; w < 3; w++)
+            if ((rc = launch_cfg(A, X, ldx, dtype, Y, ldy, K, c, s)) != TC_OK) break;
+        if (rc != TC_OK) break;
+        HIPCHECK(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; i++) launch_cfg(A, X, ldx, dtype, Y, ldy, K, c, s);
+        HIPCHECK(hipEventRecord(e1, s));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        const float us = ms * 1000.0f / reps;
+        if (us < bt) {
+            bt = us;
+            bc = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != TC_OK) return rc;
+    {
+        std::lock_guard<std::mutex> g(A->mu);
+        A->cfg[{K, dtype}] = bc;
+    }
+    if (best) *best = bc;
+    if (best_us) *best_us = bt;
+    return TC_OK;
+}
+
+int tc_spmv_batch(const tc_request *reqs, int n, void *stream)
+{
+    if (n < 0 || (n > 0 && !reqs)) return tc::fail(TC_ERR_INVALID, "tc_spmv_batch: bad argument");
+    if (n == 0) return TC_OK;
+    std::vector<WorkDesc> D((size_t)n);
+    long long units = 0;
+    for (int i = 0; i < n; i++) {
+        const tc_request &r = reqs[i];
+        int rc = check_spmv_args(r.A, r.X, r.ldx, r.dtype, r.Y, r.ldy, r.k);
+        if (rc != TC_OK) return rc;
+        if (r.k > 8) return tc::fail(TC_ERR_UNSUPPORTED, "tc_spmv_batch: k=%d > 8", r.k);
+        WorkDesc &w = D[i];
+        w.A = r.A->v;
+        w.X = r.X;
+        w.Y = r.Y;
+        w.ldx = r.ldx;
+        w.ldy = r.ldy;
+        w.K = r.k;
+        w.dtype = r.dtype;
+        w.unit_begin = (int)units;
+        w.pad0 = 0;
+        const int trw = (kWG * kBatchP) / r.k;
+        units += (r.A->v.M + trw - 1) / trw;
+    }
+    if (units > INT32_MAX) return tc::fail(TC_ERR_UNSUPPORTED, "tc_spmv_batch: too many tiles");
+    if (units == 0) return TC_OK;
+    hipStream_t s = (hipStream_t)stream;
+    WorkDesc *dd = nullptr;
+    HIPCHECK(hipMallocAsync((void **)&dd, D.size() * sizeof(WorkDesc), s));
+    HIPCHECK(hipMemcpyAsync(dd, D.data(), D.size() * sizeof(WorkDesc), hipMemcpyHostToDevice, s));
+    const int capv = 4096;
+    const int grid = (int)std::min<long long>(units, 8ll * cu_count());
+    const size_t lds = lds_bytes(capv, kBatchP, kWG * kBatchP);
+    batch_kernel<<<grid, kWG, lds, s>>>(dd, n, (int)units, capv);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipFreeAsync(dd, s));
+    // the host descriptor array must outlive the async copy
+    HIPCHECK(hipStreamSynchronize(s));
+    return TC_OK;
+}
+
+int tc_graph_capture(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int k, int iters,
+                     void *stream, tc_graph **out)
+{
+    int rc = check_spmv_args(A, X, ldx, dtype, Y, ldy, k);
+    if (rc != TC_OK) return rc;
+    if (!out || iters < 1) return tc::fail(TC_ERR_INVALID, "tc_graph_capture: bad argument");
+    hipStream_t cs = (hipStream_t)stream;
+    bool own = false;
+    if (!cs) {
+        HIPCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        own = true;
+    }
+    tc_graph *g = new tc_graph();
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int i = 0; i < iters && rc == TC_OK; i++) rc = tc_spmv(A, X, ldx, dtype, Y, ldy, k, cs);
+        hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+        if (e == hipSuccess) e = e2;
+    }
+    if (e == hipSuccess && rc == TC_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (own) (void)hipStreamDestroy(cs);
+    if (e != hipSuccess || rc != TC_OK) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return rc != TC_OK ? rc : tc::fail(TC_ERR_HIP, "tc_graph_capture: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return TC_OK;
+}
+
+int tc_graph_launch(tc_graph *g, void *stream)
+{
+    if (!g) return tc::fail(TC_ERR_INVALID, "tc_graph_launch: null graph");
+    HIPCHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+    return TC_OK;
+}
+
+int tc_graph_destroy(tc_graph *g)
+{
+    if (!g) return TC_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return TC_OK;
+}
+
+int tc_taichi_spmv(int M, int N, const TaiChi_new *T, const float *x_host, float *y_host, int warmup, int runs,
+                   float *transfer_ms, float *spmv_ms)
+{
+    if (!T || (N > 0 && !x_host) || (M > 0 && !y_host) || warmup < 0 || runs < 0)
+        return tc::fail(TC_ERR_INVALID, "tc_taichi_spmv: bad argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return tc::fail(TC_ERR_NODEVICE, "no HIP device");
+    auto t0 = std::chrono::steady_clock::now();
+    tc_matrix *A = nullptr;
+    long long nnz = (long long)T->dense_nnz + T->sparse_nnz;
+    int rc = tc_matrix_create(M, N, T, nnz, 0, nullptr, &A);
+    if (rc != TC_OK) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    if (transfer_ms) *transfer_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    float *dx = nullptr, *dy = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (dx) (void)hipFree(dx);
+        if (dy) (void)hipFree(dy);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        tc_matrix_destroy(A);
+    };
+#define STEP(expr)                                                                                          \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            cleanup();                                                                                      \
+            return tc::fail(TC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));                            \
+        }                                                                                                   \
+    } while (0)
+    STEP(hipMalloc((void **)&dx, (size_t)std::max(N, 1) * sizeof(float)));
+    STEP(hipMalloc((void **)&dy, (size_t)std::max(M, 1) * sizeof(float)));
+    if (N > 0) STEP(hipMemcpy(dx, x_host, (size_t)N * sizeof(float), hipMemcpyHostToDevice));
+    STEP(hipEventCreate(&e0));
+    STEP(hipEventCreate(&e1));
+    for (int i = 0; i < std::max(warmup, 1) && rc == TC_OK; i++) rc = tc_spmv(A, dx, 1, TC_F32, dy, 1, 1, nullptr);
+    STEP(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < runs && rc == TC_OK; i++) rc = tc_spmv(A, dx, 1, TC_F32, dy, 1, 1, nullptr);
+    STEP(hipEventRecord(e1, nullptr));
+    STEP(hipEventSynchronize(e1));
+    if (rc != TC_OK) {
+        cleanup();
+        return rc;
+    }
+    float ms = 0;
+    STEP(hipEventElapsedTime(&ms, e0, e1));
+    if (spmv_ms) *spmv_ms = runs > 0 ? ms / runs : 0.0f;
+    if (M > 0) STEP(hipMemcpy(y_host, dy, (size_t)M * sizeof(float), hipMemcpyDeviceToHost));
+#undef STEP
+    cleanup();
+    return TC_OK;
+}
+
+}  // extern "C"

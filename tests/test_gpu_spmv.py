@@ -1,0 +1,193 @@
+"""GPU parity of the hot path: thumbnail kernel (a) and the fused SpMV kernels
+(b)+(c) against the oracle, through the C ABI."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import taichi_amd as T
+from helpers import cfg1, check_float, host_format, rmat, run_spmv
+
+pytestmark = pytest.mark.gpu
+
+
+def _thumb_gpu(M, N, rp, col, dev):
+    rc, H, W, _, _ = T.thumb_dims(M, N)
+    assert rc == 0
+    rpd = torch.from_numpy(rp).to(dev)
+    cd = torch.from_numpy(col if len(col) else np.zeros(1, np.int32)).to(dev)
+    img = torch.empty(H * W, dtype=torch.uint8, device=dev)
+    T.thumbnail_gpu(M, N, rpd, cd, img)
+    torch.cuda.synchronize()
+    return img.cpu().numpy().reshape(H, W)
+
+
+@pytest.mark.parametrize("case", ["cfg1", "rmat16", "rmat20", "wide", "odd"])
+def test_thumbnail_bit_exact(case, dev):
+    if case == "cfg1":
+        M, N, rp, col = cfg1()
+    elif case == "rmat16":
+        M, N, rp, col = rmat(16)
+    elif case == "rmat20":
+        M, N, rp, col = rmat(20)
+    elif case == "wide":  # N > M: W > 512
+        M, N = 3000, 9000
+        rng = np.random.default_rng(3)
+        lens = rng.integers(0, 12, M)
+        rp = np.zeros(M + 1, np.int32)
+        rp[1:] = np.cumsum(lens)
+        col = rng.integers(0, N, int(rp[-1])).astype(np.int32)
+    else:  # non-power-of-two M (W may round to 511), rows past 512*pM dropped
+        M, N, rp, col = rmat(14, n=13001)
+    img = _thumb_gpu(M, N, rp, col, dev)
+    ref = O.thumbnail(M, N, rp, col)
+    assert np.array_equal(img, ref)
+
+
+def test_partition_shapes_gpu_matches_oracle(dev):
+    M, N, rp, col = rmat(18)
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    of, _ = O.partition(M, N, rp, col)
+    for f in O.Format.FIELDS:
+        assert getattr(fmt, f) == getattr(of, f), f
+    for f in ("xStart", "yStart", "xStop", "nDiasPtr", "neg_offsets", "csrRow", "csrCol"):
+        assert np.array_equal(getattr(fmt, f), getattr(of, f)), f
+
+
+@pytest.mark.parametrize("case", ["cfg1", "rmat18", "banded", "stencil", "small"])
+def test_spmv_integer_bit_exact(case, dev):
+    if case == "cfg1":
+        M, N, rp, col = cfg1()
+    elif case == "rmat18":
+        M, N, rp, col = rmat(18)
+    elif case == "banded":
+        M = N = 150000
+        rp, col = T.gen_banded(M, 24, 0.7, 9)
+    elif case == "stencil":
+        M = N = 300 * 300
+        rp, col = T.gen_stencil5(300, 300, 0.02, 2)
+    else:  # M <= 512: no thumbnail, plain CSR path
+        M = N = 500
+        rp, col = T.gen_rmat(500, 9, 4000, 0.25, 0.25, 0.25, 4)
+    fmt = host_format(M, N, rp, col)
+    x = T.gen_vector(N, 1, 3)
+    y, _ = run_spmv(fmt, x, dev)
+    assert np.array_equal(y, O.csr_spmv_f32(rp, col, x))
+
+
+@pytest.mark.parametrize("case", ["cfg1", "rmat18", "banded"])
+def test_spmv_random_fp64(case, dev):
+    if case == "cfg1":
+        M, N, rp, col = cfg1()
+    elif case == "rmat18":
+        M, N, rp, col = rmat(18)
+    else:
+        M = N = 150000
+        rp, col = T.gen_banded(M, 24, 0.7, 9)
+    fmt = host_format(M, N, rp, col)
+    for kind in (0, 2):
+        x = T.gen_vector(N, kind, 5)
+        y, _ = run_spmv(fmt, x, dev)
+        check_float(y, rp, col, x.astype(np.float64))
+
+
+def test_long_rows_wavefront(dev):
+    """Rows far above the long-row threshold take the wavefront path."""
+    n = 4096
+    rng = np.random.default_rng(0)
+    rows = []
+    for r in range(n):
+        if r % 97 == 0:
+            cols = np.unique(rng.integers(0, n, 3000))
+        else:
+            cols = np.unique(rng.integers(0, n, rng.integers(0, 9)))
+        rows.append(np.sort(cols))
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    fmt = host_format(n, n, rp, col)
+    x = T.gen_vector(n, 1, 8)
+    y, A = run_spmv(fmt, x, dev)
+    assert A.stats()["long_rows"] > 0
+    assert np.array_equal(y, O.csr_spmv_f32(rp, col, x))
+    xr = T.gen_vector(n, 2, 9)
+    y, _ = run_spmv(fmt, xr, dev)
+    check_float(y, rp, col, xr.astype(np.float64))
+
+
+@pytest.mark.parametrize("k", [1, 3, 4, 16, 64])
+def test_multi_rhs(k, dev):
+    M, N, rp, col = rmat(16)
+    fmt = host_format(M, N, rp, col)
+    X = T.gen_vector(N * k, 2, 10 + k).reshape(N, k)
+    Y, _ = run_spmv(fmt, X, dev)
+    Y64 = O.csr_spmm_f64(rp, col, X.astype(np.float64))
+    absX = O.csr_spmm_f64(rp, col, np.abs(X.astype(np.float64)))
+    den = np.maximum(np.abs(Y64), absX)
+    den[den == 0] = 1
+    assert float(np.max(np.abs(Y - Y64) / den)) <= 1e-6
+    # every column equals the single-vector result bit for bit
+    y0, _ = run_spmv(fmt, np.ascontiguousarray(X[:, 0]), dev)
+    assert np.array_equal(Y[:, 0], y0)
+
+
+@pytest.mark.parametrize("dt", [torch.bfloat16, torch.float16])
+def test_half_storage(dt, dev):
+    M, N, rp, col = rmat(16)
+    fmt = host_format(M, N, rp, col)
+    x = T.gen_vector(N, 2, 21)
+    xq = torch.from_numpy(x).to(dt).float().numpy()  # RNE quantised x
+    y, _ = run_spmv(fmt, x, dev, dtype=dt)
+    check_float(y, rp, col, xq.astype(np.float64))
+
+
+def test_batch_matches_alone(dev):
+    mats = []
+    for i, (scale, ef) in enumerate([(12, 4), (14, 16), (16, 8), (13, 32)]):
+        n = (1 << scale) - 37 * i
+        rp, col = T.gen_rmat(n, scale, ef * n, 0.499, 0.001, 0.001, 100 + i)
+        mats.append((n, rp, col, host_format(n, n, rp, col)))
+    items, alone = [], []
+    for i, (n, rp, col, fmt) in enumerate(mats):
+        k = [1, 2, 5, 8][i]
+        X = torch.from_numpy(T.gen_vector(n * k, 0, 200 + i).reshape(n, k)).to(dev)
+        if i == 1:
+            X = X.to(torch.bfloat16)
+        A = T.Matrix(fmt)
+        Yb = torch.full((n, k), float("nan"), device=dev)
+        Ya = torch.full((n, k), float("nan"), device=dev)
+        A.spmv(X, Ya)
+        items.append((A, X, Yb))
+        alone.append(Ya)
+    T.spmv_batch(items)
+    torch.cuda.synchronize()
+    for (A, X, Yb), Ya in zip(items, alone):
+        assert torch.equal(Yb, Ya)
+
+
+def test_graph_replay_matches_eager(dev):
+    M, N, rp, col = rmat(16)
+    fmt = host_format(M, N, rp, col)
+    A = T.Matrix(fmt)
+    x = torch.from_numpy(T.gen_vector(N, 0, 4)).to(dev)
+    ye = torch.empty(M, device=dev)
+    A.spmv(x, ye)
+    yg = torch.full((M,), float("nan"), device=dev)
+    s = torch.cuda.Stream()
+    g = T.Graph(A, x, yg, iters=5, stream=s.cuda_stream)
+    for rep in range(3):
+        yg.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.launch(stream=s.cuda_stream)
+        s.synchronize()
+        assert torch.equal(yg, ye), rep
+        torch.cuda.synchronize()
+
+
+def test_taichi_spmv_host_path(dev):
+    M, N, rp, col = cfg1()
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    x = np.ones(N, np.float32)  # main.cu:54-55
+    y, t_ms, s_ms = T.taichi_spmv(fmt, x, warmup=2, runs=5)
+    assert np.array_equal(y, O.csr_spmv_f32(rp, col, x))
+    assert s_ms > 0
