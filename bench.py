@@ -63,6 +63,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--config", default="",
+                    help="rows_per_tile,lds_values,grid[,flags] (skips the autotuner)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, run 5 plain 256 MiB device copies (profiler calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); "
@@ -89,9 +93,14 @@ def main():
     xd = torch.from_numpy(x).to(dev)
     yd = torch.empty(n, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
-    if args.autotune:
+    if args.config:
+        r, c, g, fl = (list(int(v) for v in args.config.split(",")) + [0])[:4]
+        A.set_config(1, T.F32, r, c, g, fl)
+        log(f"config: rows_per_tile={r} lds_values={c} grid={g} flags={fl}")
+    elif args.autotune:
         cfg, us = A.autotune(xd, yd)
-        log(f"autotune: rows_per_tile={cfg[0]} lds_values={cfg[1]} grid={cfg[2]} -> {us:.2f} us")
+        log(f"autotune: rows_per_tile={cfg[0]} lds_values={cfg[1]} grid={cfg[2]} "
+            f"flags={cfg[3]} -> {us:.2f} us")
 
     # correctness spot check against the FP64 CSR product
     A.spmv(xd, yd)
@@ -140,6 +149,13 @@ def main():
         cold.append(e0.elapsed_time(e1))
     ms_cold = float(np.median(cold))
     del scrub
+    if args.calibrate_copy:
+        src = torch.ones(1 << 26, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     B = A.bytes(1, T.F32)
     achieved = B / t_s / 1e9

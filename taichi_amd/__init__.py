@@ -50,7 +50,8 @@ class MatrixStats(C.Structure):
 
 
 class Config(C.Structure):
-    _fields_ = [("rows_per_tile", C.c_int), ("lds_values", C.c_int), ("grid", C.c_int)]
+    _fields_ = [("rows_per_tile", C.c_int), ("lds_values", C.c_int), ("grid", C.c_int),
+                ("flags", C.c_int)]
 
 
 class Request(C.Structure):
@@ -398,14 +399,14 @@ class Matrix:
         check(lib().tc_spmv(self.handle, X.data_ptr(), ldx, self._dtype_of(X), Y.data_ptr(), ldy,
                             k, _torch_stream(stream)), "spmv")
 
-    def set_config(self, k, dtype, rows_per_tile=0, lds_values=0, grid=0):
-        c = Config(rows_per_tile, lds_values, grid)
+    def set_config(self, k, dtype, rows_per_tile=0, lds_values=0, grid=0, flags=0):
+        c = Config(rows_per_tile, lds_values, grid, flags)
         check(lib().tc_matrix_set_config(self.handle, k, dtype, C.byref(c)), "set_config")
 
     def get_config(self, k=1, dtype=F32):
         c = Config()
         check(lib().tc_matrix_get_config(self.handle, k, dtype, C.byref(c)), "get_config")
-        return c.rows_per_tile, c.lds_values, c.grid
+        return c.rows_per_tile, c.lds_values, c.grid, c.flags
 
     def autotune(self, X, Y, stream=None):
         k = 1 if X.dim() == 1 else X.shape[1]
@@ -415,7 +416,7 @@ class Matrix:
         check(lib().tc_autotune(self.handle, X.data_ptr(), ldx, self._dtype_of(X), Y.data_ptr(),
                                 ldy, k, _torch_stream(stream), C.byref(c), C.byref(us)),
               "autotune")
-        return (c.rows_per_tile, c.lds_values, c.grid), us.value
+        return (c.rows_per_tile, c.lds_values, c.grid, c.flags), us.value
 
 
 def spmv_batch(items, stream=None):

@@ -50,6 +50,7 @@ namespace {
 
 constexpr int kWG = 256;
 constexpr int kWaves = kWG / 64;
+constexpr int kStageU = 8;  // loads in flight per lane when staging streams
 
 struct alignas(16) ShapeDev {
     int xs, xe, ys, ob, oe, omin, omax, pad;
@@ -60,6 +61,7 @@ struct MatView {
     int M, N, S, L;
     int ow;        // bytes per local row offset: 1 or 2
     int ngroups;
+    ShapeDev sh0;  // copy of shapes[0]: arrives with the kernel arguments
     const ShapeDev *shapes;
     const int *offs;
     const int *gbase;
@@ -93,10 +95,34 @@ __device__ __forceinline__ float load_x<f16_t>(const f16_t *__restrict__ X, size
     return (float)X[i];
 }
 
+// Read-only format metadata (group bases, shapes, offsets, long-row tables)
+// is read through the constant address space: uniform indices then become
+// scalar loads, which do not make the vector loads behind them wait.
+template <typename T>
+__device__ __forceinline__ T cload(const T *p, int i)
+{
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
+__device__ __forceinline__ ShapeDev load_shape(const MatView &A, int s)
+{
+    const int *p = (const int *)(A.shapes + s);
+    ShapeDev h;
+    h.xs = cload(p, 0);
+    h.xe = cload(p, 1);
+    h.ys = cload(p, 2);
+    h.ob = cload(p, 3);
+    h.oe = cload(p, 4);
+    h.omin = cload(p, 5);
+    h.omax = cload(p, 6);
+    h.pad = 0;
+    return h;
+}
+
 // start of row i's short entries (i in [0, M]); row i ends at row_start(i+1)
 __device__ __forceinline__ int row_start(const MatView &A, int i)
 {
-    int b = A.gbase[i >> 6];
+    int b = cload(A.gbase, i >> 6);
     if (i & 63) b += A.ow == 1 ? (int)((const uint8_t *)A.roff)[i - 1] : (int)((const uint16_t *)A.roff)[i - 1];
     return b;
 }
@@ -114,6 +140,19 @@ __device__ __forceinline__ float wave_sum(float v)
            __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
 }
 
+// Compensated (Neumaier) accumulation: s + c carries the running sum with
+// the rounding error of every add kept in c. Dense diagonals add x over
+// whole segments and the remainder subtracts it again at their zero cells,
+// so plain FP32 sums lose accuracy to cancellation; the compensated sum
+// keeps each y within a few ulps of the exact sum, at no cost in a
+// memory-bound kernel. Integer-valued inputs stay exact (c stays 0).
+__device__ __forceinline__ void cadd(float &s, float &c, float x)
+{
+    const float t = s + x;
+    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+}
+
 // Bijection of block ids so that each XCD (blocks are dispatched round-robin
 // over the 8 XCDs) gets a contiguous range of slots, hence contiguous rows
 // and x windows in its own L2.
@@ -123,6 +162,163 @@ __device__ __forceinline__ int xcd_slot(int b, int G)
     return x * q + min(x, r) + l;
 }
 
+// Contiguous range [*b, *e) of `total` work units for this block. With one
+// block per unit it is just the XCD-remapped block id; otherwise a 32-bit
+// split (callers keep total * G below 2^31).
+__device__ __forceinline__ void block_units(int total, int *b, int *e)
+{
+    const int G = gridDim.x;
+    const int v = xcd_slot(blockIdx.x, G);
+    if (G == total) {
+        *b = v;
+        *e = v + 1;
+    } else {
+        *b = (int)((unsigned)v * (unsigned)total / (unsigned)G);
+        *e = (int)((unsigned)(v + 1) * (unsigned)total / (unsigned)G);
+    }
+}
+
+// Stage the signed x values of ne contiguous entries (K columns each) into
+// s_val[j*K + c]. The loads of each batch (U items per lane) are all issued
+// before the first gather waits on them; lanes past the end re-read a valid
+// item and do not store.
+template <int K, typename XT, bool NT, int U = kStageU>
+__device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne, const XT *__restrict__ X, int ldx,
+                                              float *s_val)
+{
+    const int tid = threadIdx.x;
+    const int n = ne * K;
+    for (int b = 0; b < n; b += kWG * U) {
+        // slots past the end of the stream are skipped as a whole (the
+        // condition is uniform across the workgroup)
+        int v[U], cc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            v[u] = 1;
+            cc[u] = 0;
+            if (b + u * kWG < n) {
+                const int idx = min(b + u * kWG + tid, n - 1);
+                const int j = idx / K;
+                cc[u] = idx - j * K;
+                v[u] = NT ? __builtin_nontemporal_load(cp + j) : cp[j];
+            }
+        }
+        float xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) xv[u] = b + u * kWG < n ? load_x<XT>(X, (size_t)(v[u] >> 1) * ldx + cc[u]) : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int idx = b + u * kWG + tid;
+            if (idx < n) s_val[idx] = (v[u] & 1) ? xv[u] : -xv[u];
+        }
+    }
+}
+
+// Diagonal terms of one shape for the tile's pairs, in canonical order
+// (offsets as stored), added to (acc, cmp). The x window of a shape with two
+// or more offsets is staged in LDS once per tile (s_val, left free after).
+template <int K, typename XT, int P>
+__device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
+                                               const ShapeDev &sh, int ra, int rb, const int (&prow)[P],
+                                               const int (&pcol)[P], const bool (&pv)[P], float (&acc)[P],
+                                               float (&cmp)[P], float *s_val, int capv)
+{
+    const int tid = threadIdx.x;
+    const int shift = sh.ys - sh.xs;
+    const int clo = sh.ys, chi = sh.ys + (sh.xe - sh.xs);
+    const int w0 = max(clo, ra + shift + sh.omin);
+    const int w1 = min(chi, rb - 1 + shift + sh.omax + 1);
+    const bool stage = (sh.oe - sh.ob) >= 2 && w1 > w0 && (w1 - w0) * K <= capv;
+    if (stage) {
+        const int nw = (w1 - w0) * K;
+        for (int b = 0; b < nw; b += kWG * kStageU) {
+            float t[kStageU];
+#pragma unroll
+            for (int u = 0; u < kStageU; u++) {
+                const int idx = min(b + u * kWG + tid, nw - 1);
+                const int cc = idx / K, c = idx - cc * K;
+                t[u] = load_x<XT>(X, (size_t)(w0 + cc) * ldx + c);
+            }
+#pragma unroll
+            for (int u = 0; u < kStageU; u++) {
+                const int idx = b + u * kWG + tid;
+                if (idx < nw) s_val[idx] = t[u];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int r = u0 + prow[p];
+        if (!pv[p] || r < ra || r >= rb) continue;
+        const int base = r + shift;
+        float a = acc[p], e = cmp[p];
+        for (int n = sh.ob; n < sh.oe; n++) {
+            const int c = base + cload(A.offs, n);
+            if (c >= clo && c < chi)
+                cadd(a, e, stage ? s_val[(c - w0) * K + pcol[p]] : load_x<XT>(X, (size_t)c * ldx + pcol[p]));
+        }
+        acc[p] = a;
+        cmp[p] = e;
+    }
+    if (stage) __syncthreads();
+}
+
+template <int K, typename XT, int P>
+__device__ __forceinline__ void diag_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
+                                           const int (&prow)[P], const int (&pcol)[P], const bool (&pv)[P],
+                                           float (&acc)[P], float (&cmp)[P], float *s_val, int capv)
+{
+    for (int s = 0; s < A.S; s++) {
+        const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
+        const int ra = max(u0, sh.xs), rb = min(u0 + rows, sh.xe);
+        if (ra >= rb) continue;
+        diag_one_shape<K, XT, P>(A, X, ldx, u0, s, sh, ra, rb, prow, pcol, pv, acc, cmp, s_val, capv);
+    }
+}
+
+// Long rows of the tile: one wavefront each, partial sums strided over the
+// 64 lanes and reduced by wave_sum; results land in s_lsum[local_row*K + c]
+// and are added to acc. Uniform control flow; leaves s_lsum free.
+template <int K, typename XT, int P>
+__device__ __forceinline__ void long_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
+                                           const int (&lq)[P], const bool (&pv)[P], float (&acc)[P],
+                                           float (&cmp)[P], float *s_lsum, int nlsum)
+{
+    if (A.L == 0) return;
+    const int lb = cload(A.glong, u0 >> 6), le = cload(A.glong, ((u0 + rows - 1) >> 6) + 1);
+    if (lb >= le) return;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < nlsum; q += kWG) s_lsum[q] = 0.0f;
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int l = lb + wave; l < le; l += kWaves) {
+        const int row = cload(A.lrow, l);
+        if (row < u0 || row >= u0 + rows) continue;
+        const int j0 = cload(A.lptr, l), j1 = cload(A.lptr, l + 1);
+        for (int c = 0; c < K; c++) {
+            float part = 0.0f, pe = 0.0f;
+            for (int j = j0 + lane; j < j1; j += 64) {
+                const int v = A.cols[j];
+                const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
+                cadd(part, pe, (v & 1) ? xv : -xv);
+            }
+            part = wave_sum(part + pe);
+            if (lane == 0) s_lsum[(row - u0) * K + c] = part;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < P; p++)
+        if (pv[p]) cadd(acc[p], cmp[p], s_lsum[lq[p]]);
+    __syncthreads();
+}
+
+// Canonical per-(row, column) order, shared by every kernel below, with
+// compensated accumulation (cadd) into (acc, cmp) and y = acc + cmp:
+//   diagonal terms (shape order, offset order);
+//   short-row entries in stored order, one lane, serially;
+//   the long-row wavefront sum (long rows have no short entries).
 template <int K, typename XT, int P>
 __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
                                          int ldy, int u0, float *s_val, int capv, int *s_start, float *s_lsum)
@@ -134,12 +330,9 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
 
     // 1. row extents
     for (int r = tid; r <= TRW; r += kWG) s_start[r] = row_start(A, min(u0 + r, A.M));
-    __syncthreads();
-    const int ebase = s_start[0];
-    const int E = s_start[TRW] - ebase;
 
-    float acc[P];
-    int prow[P], pcol[P];
+    float acc[P], cmp[P];
+    int prow[P], pcol[P], lq[P];
     bool pv[P];
 #pragma unroll
     for (int p = 0; p < P; p++) {
@@ -147,100 +340,146 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
         prow[p] = q / K;
         pcol[p] = q - prow[p] * K;
         pv[p] = q < NP && prow[p] < rows;
+        lq[p] = q < NP ? q : 0;
         acc[p] = 0.0f;
+        cmp[p] = 0.0f;
     }
 
-    // 2. short rows: stream entries, stage signed x values, serial row sums
+    // 2. diagonal segments
+    diag_phase<K, XT, P>(A, X, ldx, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+    __syncthreads();
+    const int ebase = s_start[0];
+    const int E = s_start[TRW] - ebase;
+
+    // 3. short rows: stream entries, stage signed x values, serial row sums
     const int cape = capv / K;
     for (int e0 = 0; e0 < E; e0 += cape) {
         const int ne = min(cape, E - e0);
-        const int *__restrict__ cp = A.cols + ebase + e0;
-        for (int idx = tid; idx < ne * K; idx += kWG) {
-            const int j = idx / K, c = idx - j * K;
-            const int v = cp[j];
-            const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
-            s_val[idx] = (v & 1) ? xv : -xv;
-        }
+        stage_entries<K, XT, false>(A.cols + ebase + e0, ne, X, ldx, s_val);
         __syncthreads();
 #pragma unroll
         for (int p = 0; p < P; p++) {
             if (!pv[p]) continue;
             const int lo = max(s_start[prow[p]] - ebase, e0);
             const int hi = min(s_start[prow[p] + 1] - ebase, e0 + ne);
-            float a = acc[p];
-            for (int j = lo; j < hi; j++) a += s_val[(j - e0) * K + pcol[p]];
+            float a = acc[p], e = cmp[p];
+            for (int j = lo; j < hi; j++) cadd(a, e, s_val[(j - e0) * K + pcol[p]]);
             acc[p] = a;
+            cmp[p] = e;
         }
         __syncthreads();
     }
 
-    // 3. long rows: one wavefront per row
-    if (A.L > 0) {
-        const int lb = A.glong[u0 >> 6], le = A.glong[((u0 + rows - 1) >> 6) + 1];
-        if (lb < le) {
-            for (int q = tid; q < NP; q += kWG) s_lsum[q] = 0.0f;
-            __syncthreads();
-            const int wave = tid >> 6, lane = tid & 63;
-            for (int l = lb + wave; l < le; l += kWaves) {
-                const int row = A.lrow[l];
-                if (row < u0 || row >= u0 + rows) continue;
-                const int j0 = A.lptr[l], j1 = A.lptr[l + 1];
-                for (int c = 0; c < K; c++) {
-                    float part = 0.0f;
-                    for (int j = j0 + lane; j < j1; j += 64) {
-                        const int v = A.cols[j];
-                        const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
-                        part += (v & 1) ? xv : -xv;
-                    }
-                    part = wave_sum(part);
-                    if (lane == 0) s_lsum[(row - u0) * K + c] = part;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int p = 0; p < P; p++)
-                if (pv[p]) acc[p] += s_lsum[tid + p * kWG];
-            __syncthreads();
-        }
-    }
-
-    // 4. diagonal segments
-    for (int s = 0; s < A.S; s++) {
-        const ShapeDev sh = A.shapes[s];
-        const int ra = max(u0, sh.xs), rb = min(u0 + rows, sh.xe);
-        if (ra >= rb) continue;
-        const int shift = sh.ys - sh.xs;
-        const int clo = sh.ys, chi = sh.ys + (sh.xe - sh.xs);
-        const int w0 = max(clo, ra + shift + sh.omin);
-        const int w1 = min(chi, rb - 1 + shift + sh.omax + 1);
-        const bool stage = (sh.oe - sh.ob) >= 2 && w1 > w0 && (w1 - w0) * K <= capv;
-        if (stage) {
-            for (int idx = tid; idx < (w1 - w0) * K; idx += kWG) {
-                const int cc = idx / K, c = idx - cc * K;
-                s_val[idx] = load_x<XT>(X, (size_t)(w0 + cc) * ldx + c);
-            }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            const int r = u0 + prow[p];
-            if (!pv[p] || r < ra || r >= rb) continue;
-            const int base = r + shift;
-            float a = acc[p];
-            for (int n = sh.ob; n < sh.oe; n++) {
-                const int c = base + A.offs[n];
-                if (c >= clo && c < chi)
-                    a += stage ? s_val[(c - w0) * K + pcol[p]] : load_x<XT>(X, (size_t)c * ldx + pcol[p]);
-            }
-            acc[p] = a;
-        }
-        if (stage) __syncthreads();
-    }
+    // 4. long rows: one wavefront per row
+    long_phase<K, XT, P>(A, X, ldx, u0, rows, lq, pv, acc, cmp, s_lsum, NP);
 
     // 5. store
 #pragma unroll
     for (int p = 0; p < P; p++)
-        if (pv[p]) Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]] = acc[p];
+        if (pv[p]) Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]] = acc[p] + cmp[p];
+}
+
+// ---- single right-hand side: lane-strided row kernel ----
+// A workgroup handles a tile of 256*RPT rows; lane l of wavefront w owns the
+// rows u0 + 256*j + 64*w + l (j < RPT), so every row-indexed access -- local
+// offsets, diagonal x, y -- is one coalesced dword per lane, a wavefront's 64
+// rows form exactly one offset group (their base is one scalar load), and a
+// row's first entry is its left neighbour's end. The tile's entry range is
+// two scalar loads (tiles are 64-row aligned). All loads of a phase are
+// issued before the first use. Same canonical order as run_tile.
+template <typename XT, int RPT, int OW, bool FAST, bool LONG>
+__global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__restrict__ X, float *__restrict__ Y,
+                                                        int ntiles, int capv)
+{
+    constexpr int TRW = kWG * RPT;
+    extern __shared__ float smem[];
+    float *s_val = smem;
+    float *s_lsum = smem + capv;
+    const int tid = threadIdx.x, lane = tid & 63;
+    int tb, te;
+    block_units(ntiles, &tb, &te);
+    const uint8_t *ro8 = (const uint8_t *)A.roff;
+    const uint16_t *ro16 = (const uint16_t *)A.roff;
+    for (int t = tb; t < te; t++) {
+        const int u0 = t * TRW;
+        const int rows = min(TRW, A.M - u0);
+        const int ebase = cload(A.gbase, u0 >> 6);
+        const int E = cload(A.gbase, u0 + TRW >= A.M ? A.ngroups : (u0 + TRW) >> 6) - ebase;
+
+        // Issue order: diagonal x (one-offset shape fast path), row offsets,
+        // then the column stream and its gathers; the adds follow in the
+        // canonical order once the data is in.
+        constexpr bool fast = FAST;  // one shape with one offset (host-checked)
+        int prow[RPT], pcol[RPT], lq[RPT], endv[RPT], gbv[RPT];
+        bool pv[RPT], dok[RPT];
+        float acc[RPT], cmp[RPT], dx[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            prow[j] = j * kWG + tid;
+            pcol[j] = 0;
+            lq[j] = prow[j];
+            acc[j] = 0.0f;
+            cmp[j] = 0.0f;
+            const int r = u0 + prow[j];
+            pv[j] = r < A.M;
+            dok[j] = false;
+            dx[j] = 0.0f;
+            if (fast) {
+                const ShapeDev &sh = A.sh0;
+                const int c = r + sh.ys - sh.xs + sh.omin;
+                dok[j] = pv[j] && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs);
+                dx[j] = load_x<XT>(X, (size_t)(dok[j] ? c : sh.ys));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            const int rc = pv[j] ? u0 + prow[j] : A.M - 1;
+            // the wavefront's 64 rows are one group: its base is a scalar load
+            const int g = __builtin_amdgcn_readfirstlane(min((u0 + j * kWG + (tid & ~63)) >> 6, (A.M - 1) >> 6));
+            gbv[j] = cload(A.gbase, g);
+            endv[j] = OW == 1 ? (int)ro8[rc] : (int)ro16[rc];
+        }
+        if constexpr (!fast) diag_phase<1, XT, RPT>(A, X, 1, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+
+        int rs[RPT], re[RPT];
+        int e0 = 0;
+        do {
+            const int ne = min(capv, E - e0);
+            if (ne > 0) stage_entries<1, XT, false, 4>(A.cols + ebase + e0, ne, X, 1, s_val);
+            if (e0 == 0) {
+#pragma unroll
+                for (int j = 0; j < RPT; j++) {
+                    if (dok[j]) cadd(acc[j], cmp[j], dx[j]);
+                    // a row starts where its left neighbour ends (DPP
+                    // wave_shr:1); lane 0 starts the group
+                    const int beg = __builtin_amdgcn_update_dpp(0, endv[j], 0x138, 0xF, 0xF, false);
+                    rs[j] = gbv[j] - ebase + (lane == 0 ? 0 : beg);
+                    re[j] = pv[j] ? gbv[j] - ebase + endv[j] : rs[j];
+                }
+            }
+            if (ne > 0) {
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < RPT; j++) {
+                    const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
+                    float a = acc[j], e = cmp[j];
+                    for (int k = lo; k < hi; k++) cadd(a, e, s_val[k - e0]);
+                    acc[j] = a;
+                    cmp[j] = e;
+                }
+                __syncthreads();
+            }
+            e0 += capv;
+        } while (e0 < E);
+
+        if constexpr (LONG) long_phase<1, XT, RPT>(A, X, 1, u0, rows, lq, pv, acc, cmp, s_lsum, TRW);
+
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            if (!pv[j]) continue;
+            Y[u0 + prow[j]] = acc[j] + cmp[j];
+        }
+    }
 }
 
 // LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
@@ -258,9 +497,8 @@ __global__ __launch_bounds__(kWG) void spmv_kernel(MatView A, const XT *__restri
     float *s_val = smem;
     float *s_lsum = s_val + capv;
     int *s_start = (int *)(s_lsum + kWG * P);
-    const int G = gridDim.x;
-    const int v = xcd_slot(blockIdx.x, G);
-    const int ub = (int)((long long)v * nunits / G), ue = (int)((long long)(v + 1) * nunits / G);
+    int ub, ue;
+    block_units(nunits, &ub, &ue);
     for (int u = ub; u < ue; u++) {
         run_tile<K, XT, P>(A, X, ldx, Y, ldy, u * TRW, s_val, capv, s_start, s_lsum);
         __syncthreads();
@@ -309,9 +547,8 @@ __global__ __launch_bounds__(kWG) void batch_kernel(const WorkDesc *__restrict__
     float *s_val = smem;
     float *s_lsum = s_val + capv;
     int *s_start = (int *)(s_lsum + kWG * kBatchP);
-    const int G = gridDim.x;
-    const int v = xcd_slot(blockIdx.x, G);
-    const int ub = (int)((long long)v * total_units / G), ue = (int)((long long)(v + 1) * total_units / G);
+    int ub, ue;
+    block_units(total_units, &ub, &ue);
     if (ub >= ue) return;
     int lo = 0, hi = ndesc - 1;  // last descriptor with unit_begin <= ub
     while (lo < hi) {
@@ -395,10 +632,63 @@ int pick_k(int k)
 tc_config default_config(int K)
 {
     tc_config c{};
-    c.rows_per_tile = (kWG * 2) / K;  // P = 2
+    c.rows_per_tile = K == 1 ? 2048 : (kWG * 2) / K;  // K=1: 8 rows per thread; else P = 2
     c.lds_values = 4096;
     c.grid = 0;
+    c.flags = 0;
     return c;
+}
+
+bool rows_per_tile_ok(int K, int r)
+{
+    if (K == 1) return r == 512 || r == 1024 || r == 2048 || r == 4096;
+    for (int P : {1, 2, 4})
+        if ((kWG * P) / K == r) return true;
+    return false;
+}
+
+template <typename XT, int RPT, int OW, bool FAST, bool LONG>
+void launch_rows_k(const MatView &A, const void *X, float *Y, int ntiles, int capv, int G, size_t lds, hipStream_t s)
+{
+    spmv_rows_kernel<XT, RPT, OW, FAST, LONG><<<G, kWG, lds, s>>>(A, (const XT *)X, Y, ntiles, capv);
+}
+
+template <typename XT, int RPT>
+int launch_rows_t(const MatView &A, const void *X, float *Y, int capv, int grid, hipStream_t s)
+{
+    constexpr int TRW = kWG * RPT;
+    const int ntiles = (A.M + TRW - 1) / TRW;
+    if (ntiles == 0) return TC_OK;
+    const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
+    const size_t lds = ((size_t)capv + (A.L > 0 ? TRW : 0)) * 4;
+    const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1;
+    const bool lng = A.L > 0;
+    // specialised on offset width, the one-shape-one-offset case and long
+    // rows, so the common case carries no registers for the general paths
+    if (A.ow == 1) {
+        if (fast) lng ? launch_rows_k<XT, RPT, 1, true, true>(A, X, Y, ntiles, capv, G, lds, s)
+                      : launch_rows_k<XT, RPT, 1, true, false>(A, X, Y, ntiles, capv, G, lds, s);
+        else lng ? launch_rows_k<XT, RPT, 1, false, true>(A, X, Y, ntiles, capv, G, lds, s)
+                 : launch_rows_k<XT, RPT, 1, false, false>(A, X, Y, ntiles, capv, G, lds, s);
+    } else {
+        if (fast) lng ? launch_rows_k<XT, RPT, 2, true, true>(A, X, Y, ntiles, capv, G, lds, s)
+                      : launch_rows_k<XT, RPT, 2, true, false>(A, X, Y, ntiles, capv, G, lds, s);
+        else lng ? launch_rows_k<XT, RPT, 2, false, true>(A, X, Y, ntiles, capv, G, lds, s)
+                 : launch_rows_k<XT, RPT, 2, false, false>(A, X, Y, ntiles, capv, G, lds, s);
+    }
+    HIPCHECK(hipGetLastError());
+    return TC_OK;
+}
+
+template <typename XT>
+int launch_rows(const MatView &A, const void *X, float *Y, const tc_config &c, int capv, hipStream_t s)
+{
+    switch (c.rows_per_tile) {
+    case 512: return launch_rows_t<XT, 2>(A, X, Y, capv, c.grid, s);
+    case 1024: return launch_rows_t<XT, 4>(A, X, Y, capv, c.grid, s);
+    case 4096: return launch_rows_t<XT, 16>(A, X, Y, capv, c.grid, s);
+    default: return launch_rows_t<XT, 8>(A, X, Y, capv, c.grid, s);
+    }
 }
 
 // P from a rows_per_tile request for column count K (1, 2 or 4)
@@ -463,8 +753,14 @@ tc_config get_cfg(tc_matrix *A, int K, int dtype)
 int launch_cfg(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int K, const tc_config &c,
                hipStream_t s)
 {
-    const int P = p_for(K, c.rows_per_tile);
     const int capv = std::max(c.lds_values, 64 * K);
+    if (K == 1 && ldx == 1 && ldy == 1 && ((uintptr_t)Y & 15) == 0) {
+        if (dtype == TC_BF16) return launch_rows<bf16_t>(A->v, X, Y, c, capv, s);
+        if (dtype == TC_F16) return launch_rows<f16_t>(A->v, X, Y, c, capv, s);
+        return launch_rows<float>(A->v, X, Y, c, capv, s);
+    }
+    // generic tile kernel: K=1 configs name the rows kernel's tile; use P = 2
+    const int P = K == 1 ? 2 : p_for(K, c.rows_per_tile);
     if (dtype == TC_BF16) return launch_k<bf16_t>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
     if (dtype == TC_F16) return launch_k<f16_t>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
     return launch_k<float>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
@@ -665,6 +961,7 @@ int tc_matrix_create(int M, int N, const TaiChi_new *T, long long nnz, int long_
     v.ngroups = G;
     v.shapes = (const ShapeDev *)at(0);
     v.offs = (const int *)at(1);
+    if (S > 0) v.sh0 = shapes[0];
     v.gbase = (const int *)at(2);
     v.roff = at(3);
     v.cols = (const int *)at(4);
@@ -733,10 +1030,7 @@ int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg)
     const tc_config d = default_config(K);
     if (c.rows_per_tile <= 0) c.rows_per_tile = d.rows_per_tile;
     if (c.lds_values <= 0) c.lds_values = d.lds_values;
-    bool ok = false;
-    for (int P : {1, 2, 4})
-        if ((kWG * P) / K == c.rows_per_tile) ok = true;
-    if (!ok) return tc::fail(TC_ERR_INVALID, "rows_per_tile %d not available for k=%d", c.rows_per_tile, K);
+    if (!rows_per_tile_ok(K, c.rows_per_tile)) return tc::fail(TC_ERR_INVALID, "rows_per_tile %d not available for k=%d", c.rows_per_tile, K);
     if (c.lds_values < 512 || c.lds_values > 8192)
         return tc::fail(TC_ERR_INVALID, "lds_values %d outside [512, 8192]", c.lds_values);
     if (c.grid < 0) return tc::fail(TC_ERR_INVALID, "grid < 0");
@@ -764,13 +1058,18 @@ int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int l
     HIPCHECK(hipEventCreate(&e1));
     const int ncu = cu_count();
     std::vector<tc_config> cands;
-    for (int P : {1, 2, 4})
+    std::vector<int> tiles;
+    if (K == 1) tiles = {512, 1024, 2048, 4096};
+    else
+        for (int P : {1, 2, 4}) tiles.push_back((kWG * P) / K);
+    for (int rpt : tiles)
         for (int capv : {2048, 4096, 8192})
-            for (int grid : {0, 4 * ncu, 8 * ncu}) {
-                if (capv < 64 * K) continue;
-                tc_config c{(kWG * P) / K, capv, grid};
-                cands.push_back(c);
-            }
+            for (int grid : {0, 4 * ncu, 8 * ncu})
+                for (int flags : {0, 1}) {
+                    if (capv < 64 * K || (flags && K != 1)) continue;
+                    tc_config c{rpt, capv, grid, flags};
+                    cands.push_back(c);
+                }
     tc_config bc = default_config(K);
     float bt = 1e30f;
     const int reps = 20;
