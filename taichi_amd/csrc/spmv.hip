@@ -395,6 +395,7 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
     extern __shared__ float smem[];
     float *s_val = smem;
     float *s_lsum = smem + capv;
+    float *s_win = smem + capv + (LONG ? TRW : 0);
     const int tid = threadIdx.x, lane = tid & 63;
     int tb, te;
     block_units(ntiles, &tb, &te);
@@ -406,13 +407,12 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
         const int ebase = cload(A.gbase, u0 >> 6);
         const int E = cload(A.gbase, u0 + TRW >= A.M ? A.ngroups : (u0 + TRW) >> 6) - ebase;
 
-        // Issue order: diagonal x (one-offset shape fast path), row offsets,
-        // then the column stream and its gathers; the adds follow in the
-        // canonical order once the data is in.
-        constexpr bool fast = FAST;  // one shape with one offset (host-checked)
+        // Issue order: x window (FAST), row offsets, the first batch of the
+        // column stream; the adds follow in the canonical order.
+        constexpr int U = 4;
         int prow[RPT], pcol[RPT], lq[RPT], endv[RPT], gbv[RPT];
-        bool pv[RPT], dok[RPT];
-        float acc[RPT], cmp[RPT], dx[RPT];
+        bool pv[RPT];
+        float acc[RPT], cmp[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             prow[j] = j * kWG + tid;
@@ -420,16 +420,16 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
             lq[j] = prow[j];
             acc[j] = 0.0f;
             cmp[j] = 0.0f;
-            const int r = u0 + prow[j];
-            pv[j] = r < A.M;
-            dok[j] = false;
-            dx[j] = 0.0f;
-            if (fast) {
-                const ShapeDev &sh = A.sh0;
-                const int c = r + sh.ys - sh.xs + sh.omin;
-                dok[j] = pv[j] && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs);
-                dx[j] = load_x<XT>(X, (size_t)(dok[j] ? c : sh.ys));
-            }
+            pv[j] = u0 + prow[j] < A.M;
+        }
+        // FAST (one shape, one offset): x[w0 + i], i < TRW, holds this tile's
+        // diagonal terms; it is loaded once, kept in LDS (s_win) and also
+        // serves the remainder gathers that fall inside it.
+        const int w0 = FAST ? u0 + A.sh0.ys - A.sh0.xs + A.sh0.omin : 0;
+        float xw[RPT];
+        if constexpr (FAST) {
+#pragma unroll
+            for (int j = 0; j < RPT; j++) xw[j] = load_x<XT>(X, (size_t)min(max(w0 + prow[j], 0), A.N - 1));
         }
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
@@ -439,38 +439,72 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
             gbv[j] = cload(A.gbase, g);
             endv[j] = OW == 1 ? (int)ro8[rc] : (int)ro16[rc];
         }
-        if constexpr (!fast) diag_phase<1, XT, RPT>(A, X, 1, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+        const int ne0 = min(capv, E);
+        int cv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) cv[u] = u * kWG < ne0 ? A.cols[ebase + min(u * kWG + tid, ne0 - 1)] : 1;
 
+        if constexpr (FAST) {
+            const ShapeDev &sh = A.sh0;
+#pragma unroll
+            for (int j = 0; j < RPT; j++) {
+                s_win[prow[j]] = xw[j];
+                const int r = u0 + prow[j], c = w0 + prow[j];
+                if (pv[j] && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs))
+                    cadd(acc[j], cmp[j], xw[j]);
+            }
+            __syncthreads();
+        } else {
+            diag_phase<1, XT, RPT>(A, X, 1, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+        }
+
+        // a row starts where its left neighbour ends (DPP wave_shr:1); lane 0
+        // starts the group
         int rs[RPT], re[RPT];
-        int e0 = 0;
-        do {
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            const int beg = __builtin_amdgcn_update_dpp(0, endv[j], 0x138, 0xF, 0xF, false);
+            rs[j] = gbv[j] - ebase + (lane == 0 ? 0 : beg);
+            re[j] = pv[j] ? gbv[j] - ebase + endv[j] : rs[j];
+        }
+
+        for (int e0 = 0; e0 < E; e0 += capv) {
             const int ne = min(capv, E - e0);
-            if (ne > 0) stage_entries<1, XT, false, 4>(A.cols + ebase + e0, ne, X, 1, s_val);
-            if (e0 == 0) {
+            for (int b = 0; b < ne; b += kWG * U) {
+                int vv[U];
 #pragma unroll
-                for (int j = 0; j < RPT; j++) {
-                    if (dok[j]) cadd(acc[j], cmp[j], dx[j]);
-                    // a row starts where its left neighbour ends (DPP
-                    // wave_shr:1); lane 0 starts the group
-                    const int beg = __builtin_amdgcn_update_dpp(0, endv[j], 0x138, 0xF, 0xF, false);
-                    rs[j] = gbv[j] - ebase + (lane == 0 ? 0 : beg);
-                    re[j] = pv[j] ? gbv[j] - ebase + endv[j] : rs[j];
+                for (int u = 0; u < U; u++) {
+                    vv[u] = 1;
+                    if (e0 == 0 && b == 0) vv[u] = cv[u];
+                    else if (b + u * kWG < ne) vv[u] = A.cols[ebase + e0 + min(b + u * kWG + tid, ne - 1)];
+                }
+                float xv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    xv[u] = 0.0f;
+                    if (b + u * kWG < ne) {
+                        const int col = vv[u] >> 1;
+                        if (FAST && (unsigned)(col - w0) < (unsigned)TRW) xv[u] = s_win[col - w0];
+                        else xv[u] = load_x<XT>(X, (size_t)col);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int idx = b + u * kWG + tid;
+                    if (idx < ne) s_val[idx] = (vv[u] & 1) ? xv[u] : -xv[u];
                 }
             }
-            if (ne > 0) {
-                __syncthreads();
+            __syncthreads();
 #pragma unroll
-                for (int j = 0; j < RPT; j++) {
-                    const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
-                    float a = acc[j], e = cmp[j];
-                    for (int k = lo; k < hi; k++) cadd(a, e, s_val[k - e0]);
-                    acc[j] = a;
-                    cmp[j] = e;
-                }
-                __syncthreads();
+            for (int j = 0; j < RPT; j++) {
+                const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
+                float a = acc[j], e = cmp[j];
+                for (int k = lo; k < hi; k++) cadd(a, e, s_val[k - e0]);
+                acc[j] = a;
+                cmp[j] = e;
             }
-            e0 += capv;
-        } while (e0 < E);
+            __syncthreads();
+        }
 
         if constexpr (LONG) long_phase<1, XT, RPT>(A, X, 1, u0, rows, lq, pv, acc, cmp, s_lsum, TRW);
 
@@ -660,8 +694,8 @@ int launch_rows_t(const MatView &A, const void *X, float *Y, int capv, int grid,
     const int ntiles = (A.M + TRW - 1) / TRW;
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
-    const size_t lds = ((size_t)capv + (A.L > 0 ? TRW : 0)) * 4;
     const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1;
+    const size_t lds = ((size_t)capv + (A.L > 0 ? TRW : 0) + (fast ? TRW : 0)) * 4;
     const bool lng = A.L > 0;
     // specialised on offset width, the one-shape-one-offset case and long
     // rows, so the common case carries no registers for the general paths
