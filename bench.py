@@ -54,6 +54,141 @@ def read_traffic(name):
     return None
 
 
+def _time_events(fn, reps, stream):
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_batched(dev):
+    """256 uneven R-MAT matrices (seeds 100..355): one persistent launch over
+    all of them vs 256 plain launches, same x and y buffers."""
+    import torch
+    import taichi_amd as T
+    from taichi_amd import workloads as W
+    t0 = time.time()
+    mats = W.uneven_rmat()
+    items, nnz = [], 0
+    for (n, _, rp, col, s) in mats:
+        fmt, _ = T.partition_shapes(n, n, rp, col)
+        A = T.Matrix(fmt, nnz=len(col))
+        x = torch.from_numpy(T.gen_vector(n, 0, s)).to(dev)
+        items.append((A, x, torch.empty(n, device=dev)))
+        nnz += len(col)
+    log(f"config 4: {len(mats)} matrices, nnz={nnz}, built in {time.time() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+
+    def plain():
+        for A, x, y in items:
+            A.spmv(x, y)
+    ms_plain = _time_events(plain, 10, stream)
+    ms_batch = _time_events(lambda: T.spmv_batch(items), 10, stream)
+    # results of the batched launch equal the plain ones bit for bit
+    ref = [y.clone() for _, _, y in items]
+    plain()
+    torch.cuda.synchronize()
+    ref = [y.clone() for _, _, y in items]
+    for _, _, y in items:
+        y.fill_(float("nan"))
+    T.spmv_batch(items)
+    torch.cuda.synchronize()
+    same = all(torch.equal(r, y) for r, (_, _, y) in zip(ref, items))
+    return {"workload": "256 uneven R-MAT matrices, seeds 100..355, n=round(2^U[12,18]), ef U{4..32}",
+            "nnz_total": nnz, "plain_ms": round(ms_plain, 4), "batched_ms": round(ms_batch, 4),
+            "plain_gnnz_per_s": round(nnz / ms_plain / 1e6, 2),
+            "batched_gnnz_per_s": round(nnz / ms_batch / 1e6, 2),
+            "speedup": round(ms_plain / ms_batch, 3), "batched_equals_plain": bool(same)}
+
+
+def bench_served(dev, requests=40000, rate=20000.0):
+    """64 resident matrices (32 banded, 32 R-MAT); requests pick a matrix
+    uniformly, FP32 or BF16 x, k uniform in 1..8; Poisson arrivals at 20k
+    req/s for p50/p99, closed loop for saturation throughput, and the plain
+    one-launch-per-request baseline."""
+    import torch
+    import taichi_amd as T
+    from taichi_amd import workloads as W
+    t0 = time.time()
+    mix = W.serving_mix()
+    srv = T.Server(hbm_cap_bytes=200 << 30, max_batch=64, streams=3, max_inflight=256)
+    items, keep, nmax = [], [], 0
+    for i, (kind, n, rp, col) in enumerate(mix):
+        fmt, _ = T.partition_shapes(n, n, rp, col)
+        mid = srv.add_matrix(fmt, nnz=len(col))
+        nmax = max(nmax, n)
+        for dt, code in ((torch.float32, T.F32), (torch.bfloat16, T.BF16)):
+            X = torch.from_numpy(T.gen_vector(n * 8, 0, 7000 + i).reshape(n, 8)).to(dev).to(dt)
+            keep.append(X)
+            for k in range(1, 9):
+                items.append((mid, X.data_ptr(), 8, code, k))
+    ybufs = [torch.empty(nmax, 8, device=dev) for _ in range(512)]
+    st = srv.stats()
+    log(f"config 5: {len(mix)} matrices resident ({st['resident_bytes'] / 1e9:.2f} GB), "
+        f"built in {time.time() - t0:.1f}s")
+    srv.run_load(items, ybufs, 8, count=2000, rate=0.0, concurrency=256, seed=1)  # warm-up
+    poisson = srv.run_load(items, ybufs, 8, count=requests, rate=rate, seed=2)
+    closed = srv.run_load(items, ybufs, 8, count=requests, rate=0.0, concurrency=256, seed=3)
+    plain = srv.run_plain(items, ybufs, 8, count=max(2000, requests // 10), seed=4)
+    srv.close()
+
+    def pick(r):
+        return {k: round(r[k], 3) for k in ("req_per_s", "p50_us", "p99_us", "mean_us", "max_us",
+                                            "mean_batch")} | {"completed": r["completed"]}
+    return {"workload": "64 resident matrices (32 banded w U{3..64} density 0.7, 32 R-MAT ef 16), "
+                        "n U[2^16,2^20], x FP32 or BF16, k U{1..8}",
+            "resident_bytes": st["resident_bytes"],
+            "poisson_20k": pick(poisson), "saturation": pick(closed), "plain": pick(plain),
+            "served_vs_plain_req_per_s": round(closed["req_per_s"] / plain["req_per_s"], 3)}
+
+
+def bench_multirhs(dev, ks=(1, 4, 16, 64)):
+    """2048^2 5-point stencil + 2% uniform ones (seed 2): k right-hand sides
+    N(0,1), FP32 and BF16 storage of X, FP32 accumulation; error against the
+    FP64 oracle on the quantised X, componentwise."""
+    import torch
+    import oracle
+    import taichi_amd as T
+    from taichi_amd import workloads as W
+    M, N, rp, col = W.stencil5()
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    A = T.Matrix(fmt, nnz=len(col))
+    stream = torch.cuda.current_stream()
+    sample = np.arange(0, M, 997)  # rows checked against FP64
+    sub_rp = np.concatenate([[0], np.cumsum(rp[sample + 1] - rp[sample])]).astype(np.int32)
+    sub_col = np.concatenate([col[rp[r]:rp[r + 1]] for r in sample]).astype(np.int32)
+    out = {"workload": "2048x2048 5-point stencil + 2% uniform ones (seed 2)", "nnz": len(col),
+           "dia_shapes": fmt.dia_shapes, "runs": []}
+    for k in ks:
+        Xf = torch.randn(N, k, device=dev, generator=torch.Generator(device=dev).manual_seed(k))
+        Y = torch.empty(M, k, device=dev)
+        for dt, name in ((torch.float32, "f32"), (torch.bfloat16, "bf16")):
+            X = Xf.to(dt)
+            ms = _time_events(lambda: A.spmv(X, Y), 10, stream)
+            Xq = X.float().cpu().numpy().astype(np.float64)
+            Y64 = oracle.csr_spmm_f64(sub_rp, sub_col, Xq)
+            absY = oracle.csr_spmm_f64(sub_rp, sub_col, np.abs(Xq))
+            Ys = Y.cpu().numpy()[sample].astype(np.float64)
+            den = np.maximum(np.abs(Y64), absY)
+            den[den == 0] = 1
+            err = float(np.max(np.abs(Ys - Y64) / den))
+            Y64u = oracle.csr_spmm_f64(sub_rp, sub_col, Xf.cpu().numpy().astype(np.float64))
+            err_u = float(np.max(np.abs(Ys - Y64u) / den))
+            B = A.bytes(k, T.BF16 if name == "bf16" else T.F32)
+            out["runs"].append({"k": k, "x": name, "ms": round(ms, 4),
+                                "gnnz_per_s": round(len(col) * k / ms / 1e6, 2),
+                                "gbs": round(B / ms / 1e6, 1), "rel_err_vs_fp64": err,
+                                "rel_err_vs_fp64_unquantised_x": err_u})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +197,8 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched, served and multi-RHS sections")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--config", default="",
                     help="rows_per_tile,lds_values,grid[,flags] (skips the autotuner)")
@@ -188,6 +325,17 @@ def main():
         "max_rel_err_vs_fp64": err,
         "wall_s_timed": round(wall, 4),
     }
+    del A, xd, yd
+    if not args.no_extra:
+        log("config 4: batched uneven matrices")
+        out["batched"] = bench_batched(dev)
+        log(f"  {out['batched']}")
+        log("config 5: served mix")
+        out["served"] = bench_served(dev)
+        log(f"  {out['served']}")
+        log("config 3: multi-RHS")
+        out["multi_rhs"] = bench_multirhs(dev)
+        log(f"  {out['multi_rhs']}")
     if not args.no_cpu:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(rp, col, x, args.cpu_seconds)

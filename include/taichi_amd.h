@@ -153,6 +153,9 @@ typedef struct tc_matrix_stats {
 int tc_matrix_create(int M, int N, const TaiChi_new *t, long long nnz,
                      int long_threshold, void *stream, tc_matrix **out);
 int tc_matrix_destroy(tc_matrix *A);
+/* device bytes tc_matrix_create would allocate for this format (host-only
+   computation, no device needed; used for admission by footprint) */
+int tc_matrix_footprint(int M, int N, const TaiChi_new *t, int long_threshold, long long *bytes);
 int tc_matrix_get_stats(const tc_matrix *A, tc_matrix_stats *out);
 /* algorithmic HBM bytes of one SpMV: format arrays once, x once (k columns of
    the given storage type), y written once */
@@ -195,6 +198,67 @@ typedef struct tc_request {
 /* one persistent-kernel launch over all requests; each y is bit-identical
    to what tc_spmv computes for that request alone */
 int tc_spmv_batch(const tc_request *reqs, int n, void *stream);
+
+/* ---- serving (no reference counterpart): resident matrices, admission by
+   HBM footprint, continuous batching into persistent launches ---- */
+typedef struct tc_server tc_server;
+typedef struct tc_server_config {
+    long long hbm_cap_bytes;  /* admission cap on the resident formats' bytes */
+    int max_batch;            /* requests per launch */
+    int streams;              /* launch slots that may run concurrently */
+    int max_inflight;         /* requests on the GPU at once */
+} tc_server_config;
+typedef struct tc_server_stats {
+    long long resident_bytes;
+    int matrices, pending, inflight;
+    long long completed, batches;
+    double mean_batch;
+} tc_server_stats;
+int tc_server_create(const tc_server_config *cfg, tc_server **out);
+/* drains pending work, then frees everything, resident matrices included */
+int tc_server_destroy(tc_server *s);
+/* uploads a format if its footprint fits under the cap (TC_ERR_CAPACITY
+   otherwise, nothing allocated); *id names it in requests */
+int tc_server_add_matrix(tc_server *s, int M, int N, const TaiChi_new *t, long long nnz, int *id);
+/* TC_ERR_STATE while requests on it are pending or in flight */
+int tc_server_remove_matrix(tc_server *s, int id);
+/* Y[M x k] = A_id * X[N x k] (k in 1..8); X and Y are device buffers owned
+   by the caller and left untouched by others until the ticket completes */
+int tc_server_submit(tc_server *s, int matrix_id, const void *X, int ldx, int dtype, float *Y,
+                     int ldy, int k, unsigned long long *ticket);
+/* blocks until the ticket completes; latency = submit -> completion seen */
+int tc_server_wait(tc_server *s, unsigned long long ticket, double *latency_us);
+int tc_server_get_stats(tc_server *s, tc_server_stats *out);
+
+/* load generation over resident matrices: each request picks a template
+   uniformly (seeded) and writes y into ybufs[i % nybufs] (each large enough
+   for any request, row stride ldy) */
+typedef struct tc_load_item {
+    int matrix_id;
+    const void *X;
+    int ldx;
+    int dtype;
+    int k;
+} tc_load_item;
+typedef struct tc_load_spec {
+    const tc_load_item *items;
+    int nitems;
+    float *const *ybufs;
+    int nybufs;
+    int ldy;
+    double rate;          /* > 0: Poisson arrivals at this rate (req/s); else closed loop */
+    int concurrency;      /* closed loop: requests kept outstanding */
+    long long count;      /* requests to issue */
+    uint64_t seed;
+} tc_load_spec;
+typedef struct tc_load_result {
+    long long completed;
+    double seconds, req_per_s, p50_us, p99_us, mean_us, max_us, mean_batch;
+} tc_load_result;
+/* served: requests go through the server's continuous batching */
+int tc_server_run_load(tc_server *s, const tc_load_spec *spec, tc_load_result *out);
+/* plain baseline: one tc_spmv launch per request, each waited for */
+int tc_server_run_plain(tc_server *s, const tc_load_spec *spec, tc_load_result *out);
 
 /* ---- graphs (no reference counterpart) ---- */
 typedef struct tc_graph tc_graph;

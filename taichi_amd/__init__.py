@@ -54,6 +54,35 @@ class Config(C.Structure):
                 ("flags", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_cap_bytes", C.c_longlong), ("max_batch", C.c_int), ("streams", C.c_int),
+                ("max_inflight", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("resident_bytes", C.c_longlong), ("matrices", C.c_int), ("pending", C.c_int),
+                ("inflight", C.c_int), ("completed", C.c_longlong), ("batches", C.c_longlong),
+                ("mean_batch", C.c_double)]
+
+
+class LoadItem(C.Structure):
+    _fields_ = [("matrix_id", C.c_int), ("X", C.c_void_p), ("ldx", C.c_int), ("dtype", C.c_int),
+                ("k", C.c_int)]
+
+
+class LoadSpec(C.Structure):
+    _fields_ = [("items", C.POINTER(LoadItem)), ("nitems", C.c_int),
+                ("ybufs", C.POINTER(C.c_void_p)), ("nybufs", C.c_int), ("ldy", C.c_int),
+                ("rate", C.c_double), ("concurrency", C.c_int), ("count", C.c_longlong),
+                ("seed", C.c_uint64)]
+
+
+class LoadResult(C.Structure):
+    _fields_ = [("completed", C.c_longlong), ("seconds", C.c_double), ("req_per_s", C.c_double),
+                ("p50_us", C.c_double), ("p99_us", C.c_double), ("mean_us", C.c_double),
+                ("max_us", C.c_double), ("mean_batch", C.c_double)]
+
+
 class Request(C.Structure):
     _fields_ = [("A", C.c_void_p), ("X", C.c_void_p), ("ldx", C.c_int), ("dtype", C.c_int),
                 ("Y", C.c_void_p), ("ldy", C.c_int), ("k", C.c_int)]
@@ -116,6 +145,19 @@ def lib():
                                        vp, P(vp)]),
         "tc_graph_launch": (C.c_int, [vp, vp]),
         "tc_graph_destroy": (C.c_int, [vp]),
+        "tc_matrix_footprint": (C.c_int, [C.c_int, C.c_int, P(TaiChiNew), C.c_int,
+                                          P(C.c_longlong)]),
+        "tc_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
+        "tc_server_destroy": (C.c_int, [vp]),
+        "tc_server_add_matrix": (C.c_int, [vp, C.c_int, C.c_int, P(TaiChiNew), C.c_longlong,
+                                           P(C.c_int)]),
+        "tc_server_remove_matrix": (C.c_int, [vp, C.c_int]),
+        "tc_server_submit": (C.c_int, [vp, C.c_int, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int,
+                                       P(C.c_ulonglong)]),
+        "tc_server_wait": (C.c_int, [vp, C.c_ulonglong, P(C.c_double)]),
+        "tc_server_get_stats": (C.c_int, [vp, P(ServerStats)]),
+        "tc_server_run_load": (C.c_int, [vp, P(LoadSpec), P(LoadResult)]),
+        "tc_server_run_plain": (C.c_int, [vp, P(LoadSpec), P(LoadResult)]),
         "tc_gen_diagonals_random": (C.c_int, [C.c_int, i32, C.c_int, C.c_longlong, C.c_uint64,
                                               P(P(C.c_int)), P(P(C.c_int)), P(C.c_longlong)]),
         "tc_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_longlong, C.c_double, C.c_double,
@@ -465,3 +507,85 @@ def taichi_spmv(fmt, x, warmup=10, runs=500):
     check(lib().tc_taichi_spmv(fmt.M, fmt.N, C.byref(st), x, y, warmup, runs, C.byref(t_ms),
                                C.byref(s_ms)), "taichi_spmv")
     return y, t_ms.value, s_ms.value
+
+
+def footprint(fmt, long_threshold=0):
+    """Device bytes tc_matrix_create would allocate for fmt (host only)."""
+    st = fmt.struct()
+    b = C.c_longlong()
+    check(lib().tc_matrix_footprint(fmt.M, fmt.N, C.byref(st), long_threshold, C.byref(b)),
+          "footprint")
+    return b.value
+
+
+class Server:
+    """Continuous-batching SpMV server over resident matrices (tc_server)."""
+
+    def __init__(self, hbm_cap_bytes=200 << 30, max_batch=64, streams=3, max_inflight=256):
+        h = C.c_void_p()
+        cfg = ServerConfig(hbm_cap_bytes, max_batch, streams, max_inflight)
+        check(lib().tc_server_create(C.byref(cfg), C.byref(h)), "server_create")
+        self.handle = h
+
+    def add_matrix(self, fmt, nnz=None):
+        st = fmt.struct()
+        mid = C.c_int()
+        if nnz is None:
+            nnz = int(fmt.dense_nnz) + int(fmt.sparse_nnz)
+        check(lib().tc_server_add_matrix(self.handle, fmt.M, fmt.N, C.byref(st), nnz,
+                                         C.byref(mid)), "server_add_matrix")
+        return mid.value
+
+    def remove_matrix(self, mid):
+        check(lib().tc_server_remove_matrix(self.handle, mid), "server_remove_matrix")
+
+    def submit(self, mid, X, Y):
+        k = 1 if X.dim() == 1 else X.shape[1]
+        t = C.c_ulonglong()
+        check(lib().tc_server_submit(self.handle, mid, X.data_ptr(),
+                                     1 if X.dim() == 1 else X.stride(0), Matrix._dtype_of(X),
+                                     Y.data_ptr(), 1 if Y.dim() == 1 else Y.stride(0), k,
+                                     C.byref(t)), "server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        lat = C.c_double()
+        check(lib().tc_server_wait(self.handle, ticket, C.byref(lat)), "server_wait")
+        return lat.value
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().tc_server_get_stats(self.handle, C.byref(s)), "server_stats")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def _spec(self, items, ybufs, ldy, rate, concurrency, count, seed):
+        arr = (LoadItem * len(items))(*[LoadItem(*it) for it in items])
+        yb = (C.c_void_p * len(ybufs))(*[y.data_ptr() for y in ybufs])
+        spec = LoadSpec(arr, len(items), yb, len(ybufs), ldy, rate, concurrency, count, seed)
+        return spec, (arr, yb)
+
+    def run_load(self, items, ybufs, ldy, count, rate=0.0, concurrency=256, seed=7):
+        """items: [(matrix_id, X_ptr, ldx, dtype, k)]; returns the result dict."""
+        spec, keep = self._spec(items, ybufs, ldy, rate, concurrency, count, seed)
+        res = LoadResult()
+        check(lib().tc_server_run_load(self.handle, C.byref(spec), C.byref(res)), "run_load")
+        del keep
+        return {f: getattr(res, f) for f, _ in LoadResult._fields_}
+
+    def run_plain(self, items, ybufs, ldy, count, seed=7):
+        spec, keep = self._spec(items, ybufs, ldy, 0.0, 1, count, seed)
+        res = LoadResult()
+        check(lib().tc_server_run_plain(self.handle, C.byref(spec), C.byref(res)), "run_plain")
+        del keep
+        return {f: getattr(res, f) for f, _ in LoadResult._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().tc_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -44,6 +44,7 @@
 #include <mutex>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 
 namespace {
@@ -541,6 +542,7 @@ __global__ __launch_bounds__(kWG) void spmv_kernel(MatView A, const XT *__restri
 
 // ---- batched persistent kernel ----
 constexpr int kBatchP = 2;
+constexpr int kBatchLdsDesc = 1024;  // tile starts of up to this many requests go to LDS
 
 struct WorkDesc {
     MatView A;
@@ -581,13 +583,21 @@ __global__ __launch_bounds__(kWG) void batch_kernel(const WorkDesc *__restrict__
     float *s_val = smem;
     float *s_lsum = s_val + capv;
     int *s_start = (int *)(s_lsum + kWG * kBatchP);
+    __shared__ int s_ub[kBatchLdsDesc];
     int ub, ue;
     block_units(total_units, &ub, &ue);
     if (ub >= ue) return;
-    int lo = 0, hi = ndesc - 1;  // last descriptor with unit_begin <= ub
+    // last descriptor with unit_begin <= ub: searched in LDS (one coalesced
+    // load of the tile starts) when the batch is small enough
+    const bool in_lds = ndesc <= kBatchLdsDesc;
+    if (in_lds) {
+        for (int i = threadIdx.x; i < ndesc; i += kWG) s_ub[i] = D[i].unit_begin;
+        __syncthreads();
+    }
+    int lo = 0, hi = ndesc - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (D[mid].unit_begin <= ub) lo = mid;
+        if ((in_lds ? s_ub[mid] : D[mid].unit_begin) <= ub) lo = mid;
         else hi = mid - 1;
     }
     int d = lo;
@@ -811,6 +821,50 @@ int check_spmv_args(const tc_matrix *A, const void *X, int ldx, int dtype, const
 
 }  // namespace
 
+namespace tc {
+
+size_t batch_desc_size() { return sizeof(WorkDesc); }
+
+// Fills n descriptors in host_desc (pinned or pageable memory that outlives
+// the copy) and enqueues their copy to dev_desc and one persistent launch on
+// s. Each request's y equals what tc_spmv computes for it alone.
+int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s)
+{
+    WorkDesc *D = (WorkDesc *)host_desc;
+    long long units = 0;
+    for (int i = 0; i < n; i++) {
+        const tc_request &r = reqs[i];
+        int rc = check_spmv_args(r.A, r.X, r.ldx, r.dtype, r.Y, r.ldy, r.k);
+        if (rc != TC_OK) return rc;
+        if (r.k > 8) return tc::fail(TC_ERR_UNSUPPORTED, "batch: k=%d > 8", r.k);
+        WorkDesc &w = D[i];
+        w.A = r.A->v;
+        w.X = r.X;
+        w.Y = r.Y;
+        w.ldx = r.ldx;
+        w.ldy = r.ldy;
+        w.K = r.k;
+        w.dtype = r.dtype;
+        w.unit_begin = (int)units;
+        w.pad0 = 0;
+        const int trw = (kWG * kBatchP) / r.k;
+        units += (r.A->v.M + trw - 1) / trw;
+    }
+    if (units > INT32_MAX) return tc::fail(TC_ERR_UNSUPPORTED, "batch: too many tiles");
+    if (units == 0) return TC_OK;
+    HIPCHECK(hipMemcpyAsync(dev_desc, D, (size_t)n * sizeof(WorkDesc), hipMemcpyHostToDevice, s));
+    const int capv = 4096;
+    // one workgroup per tile: the hardware scheduler balances requests of
+    // very different sizes
+    const int grid = (int)units;
+    const size_t lds = lds_bytes(capv, kBatchP, kWG * kBatchP);
+    batch_kernel<<<grid, kWG, lds, s>>>((const WorkDesc *)dev_desc, n, (int)units, capv);
+    HIPCHECK(hipGetLastError());
+    return TC_OK;
+}
+
+}  // namespace tc
+
 extern "C" {
 
 int tc_device_count(void)
@@ -1017,6 +1071,33 @@ int tc_matrix_create(int M, int N, const TaiChi_new *T, long long nnz, int long_
     return TC_OK;
 }
 
+int tc_matrix_footprint(int M, int N, const TaiChi_new *T, int long_threshold, long long *bytes)
+{
+    if (!T || !bytes || M < 0 || N < 0 || (M > 0 && !T->csrRow))
+        return tc::fail(TC_ERR_INVALID, "tc_matrix_footprint: bad argument");
+    const int LT = long_threshold > 0 ? std::min(long_threshold, 1023) : tc::kDefaultLongRow;
+    const int S = std::max(T->dia_shapes, 0);
+    const long long D = S > 0 ? T->nDiasPtr[S] : 0;
+    const long long R = M > 0 ? T->csrRow[M] : 0;
+    const long long G = (M + 63) / 64;
+    long long L = 0, short_total = 0, gstart = 0;
+    int maxoff = 0;
+    for (int r = 0; r < M; r++) {
+        const int len = T->csrRow[r + 1] - T->csrRow[r];
+        if ((r & 63) == 0) gstart = short_total;
+        if (len > LT) L++;
+        else short_total += len;
+        maxoff = std::max(maxoff, (int)(short_total - gstart));
+    }
+    const int ow = maxoff <= 255 ? 1 : 2;
+    const long long sizes[] = {S * (long long)sizeof(ShapeDev), 4 * D, 4 * (G + 1), (long long)M * ow, 4 * R,
+                               L > 0 ? 4 * (G + 1) : 0, 4 * L, L > 0 ? 4 * (L + 1) : 0};
+    long long total = 0;
+    for (long long b : sizes) total += (b + 255) & ~255ll;
+    *bytes = total;
+    return TC_OK;
+}
+
 int tc_matrix_destroy(tc_matrix *A)
 {
     if (!A) return TC_OK;
@@ -1139,41 +1220,15 @@ int tc_spmv_batch(const tc_request *reqs, int n, void *stream)
 {
     if (n < 0 || (n > 0 && !reqs)) return tc::fail(TC_ERR_INVALID, "tc_spmv_batch: bad argument");
     if (n == 0) return TC_OK;
-    std::vector<WorkDesc> D((size_t)n);
-    long long units = 0;
-    for (int i = 0; i < n; i++) {
-        const tc_request &r = reqs[i];
-        int rc = check_spmv_args(r.A, r.X, r.ldx, r.dtype, r.Y, r.ldy, r.k);
-        if (rc != TC_OK) return rc;
-        if (r.k > 8) return tc::fail(TC_ERR_UNSUPPORTED, "tc_spmv_batch: k=%d > 8", r.k);
-        WorkDesc &w = D[i];
-        w.A = r.A->v;
-        w.X = r.X;
-        w.Y = r.Y;
-        w.ldx = r.ldx;
-        w.ldy = r.ldy;
-        w.K = r.k;
-        w.dtype = r.dtype;
-        w.unit_begin = (int)units;
-        w.pad0 = 0;
-        const int trw = (kWG * kBatchP) / r.k;
-        units += (r.A->v.M + trw - 1) / trw;
-    }
-    if (units > INT32_MAX) return tc::fail(TC_ERR_UNSUPPORTED, "tc_spmv_batch: too many tiles");
-    if (units == 0) return TC_OK;
     hipStream_t s = (hipStream_t)stream;
-    WorkDesc *dd = nullptr;
-    HIPCHECK(hipMallocAsync((void **)&dd, D.size() * sizeof(WorkDesc), s));
-    HIPCHECK(hipMemcpyAsync(dd, D.data(), D.size() * sizeof(WorkDesc), hipMemcpyHostToDevice, s));
-    const int capv = 4096;
-    const int grid = (int)std::min<long long>(units, 8ll * cu_count());
-    const size_t lds = lds_bytes(capv, kBatchP, kWG * kBatchP);
-    batch_kernel<<<grid, kWG, lds, s>>>(dd, n, (int)units, capv);
-    HIPCHECK(hipGetLastError());
+    std::vector<char> host((size_t)n * tc::batch_desc_size());
+    void *dd = nullptr;
+    HIPCHECK(hipMallocAsync(&dd, host.size(), s));
+    int rc = tc::batch_enqueue(reqs, n, host.data(), dd, s);
     HIPCHECK(hipFreeAsync(dd, s));
     // the host descriptor array must outlive the async copy
     HIPCHECK(hipStreamSynchronize(s));
-    return TC_OK;
+    return rc;
 }
 
 int tc_graph_capture(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int k, int iters,

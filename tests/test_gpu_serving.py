@@ -1,0 +1,96 @@
+"""Serving runtime: continuous batching over resident matrices. Each result
+must be bit-identical alone (tc_spmv) and in the mix; admission follows the
+HBM footprint."""
+import numpy as np
+import pytest
+import torch
+
+import taichi_amd as T
+from helpers import host_format
+from taichi_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _mats(count=6):
+    out = []
+    for i in range(count):
+        if i % 2:
+            n = 20000 + 1111 * i
+            rp, col = T.gen_banded(n, 5 + 3 * i, 0.7, 40 + i)
+        else:
+            n, _, rp, col = W.rmat(30000 + 777 * i, 8, 60 + i)
+        out.append((n, rp, col, host_format(n, n, rp, col)))
+    return out
+
+
+def test_served_results_equal_alone(dev):
+    mats = _mats()
+    srv = T.Server(hbm_cap_bytes=1 << 30, max_batch=16, streams=3, max_inflight=64)
+    ids = [srv.add_matrix(fmt) for (_, _, _, fmt) in mats]
+    alone = [T.Matrix(fmt) for (_, _, _, fmt) in mats]
+    rng = np.random.default_rng(0)
+    reqs = []
+    for q in range(120):
+        i = int(rng.integers(len(mats)))
+        n = mats[i][0]
+        k = int(rng.integers(1, 9))
+        dt = [torch.float32, torch.bfloat16, torch.float16][q % 3]
+        X = torch.from_numpy(T.gen_vector(n * k, 2, 1000 + q).reshape(n, k)).to(dev).to(dt)
+        Y = torch.full((n, k), float("nan"), device=dev)
+        reqs.append((i, X, Y, srv.submit(ids[i], X, Y)))
+    for i, X, Y, t in reqs:
+        srv.wait(t)
+    torch.cuda.synchronize()
+    for i, X, Y, t in reqs:
+        Ya = torch.full_like(Y, float("nan"))
+        alone[i].spmv(X, Ya)
+        torch.cuda.synchronize()
+        assert torch.equal(Y, Ya)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["batches"] >= 1
+    srv.close()
+
+
+def test_admission_by_footprint(dev):
+    mats = _mats(4)
+    fps = [T.footprint(fmt) for (_, _, _, fmt) in mats]
+    cap = fps[0] + fps[1] + fps[2] // 2
+    srv = T.Server(hbm_cap_bytes=cap)
+    a = srv.add_matrix(mats[0][3])
+    b = srv.add_matrix(mats[1][3])
+    assert srv.stats()["resident_bytes"] == fps[0] + fps[1]
+    with pytest.raises(T.TaiChiError) as e:
+        srv.add_matrix(mats[2][3])
+    assert e.value.status == -105  # TC_ERR_CAPACITY
+    assert srv.stats()["resident_bytes"] == fps[0] + fps[1]
+    srv.remove_matrix(a)
+    c = srv.add_matrix(mats[2][3])
+    assert srv.stats()["resident_bytes"] == fps[1] + fps[2]
+    # the footprint is what the device allocation really is
+    for (_, _, _, fmt), fp in zip(mats, fps):
+        assert T.Matrix(fmt).stats()["device_bytes"] == fp
+    del b, c
+    srv.close()
+
+
+def test_load_generator_open_and_closed(dev):
+    mats = _mats(4)
+    srv = T.Server(hbm_cap_bytes=1 << 30, max_batch=32, streams=2, max_inflight=64)
+    items, keep = [], []
+    for (n, _, _, fmt) in mats:
+        mid = srv.add_matrix(fmt)
+        for dt, code in ((torch.float32, T.F32), (torch.bfloat16, T.BF16)):
+            X = torch.rand(n, 8, device=dev).to(dt)
+            keep.append(X)
+            for k in (1, 4, 8):
+                items.append((mid, X.data_ptr(), 8, code, k))
+    nmax = max(m[0] for m in mats)
+    ybufs = [torch.empty(nmax, 8, device=dev) for _ in range(64)]
+    r = srv.run_load(items, ybufs, 8, count=400, rate=20000.0, seed=1)
+    assert r["completed"] == 400 and 0 < r["p50_us"] <= r["p99_us"] <= r["max_us"]
+    r = srv.run_load(items, ybufs, 8, count=400, rate=0.0, concurrency=48, seed=2)
+    assert r["completed"] == 400 and r["req_per_s"] > 0 and r["mean_batch"] >= 1
+    p = srv.run_plain(items, ybufs, 8, count=100, seed=3)
+    assert p["completed"] == 100
+    srv.close()
