@@ -193,10 +193,12 @@ typedef struct tc_request {
     int dtype;
     float *Y;
     int ldy;
-    int k;      /* 1..8 */
+    int k;      /* 1..64 */
 } tc_request;
-/* one persistent-kernel launch over all requests; each y is bit-identical
-   to what tc_spmv computes for that request alone */
+/* batched SpMV over n requests on any mix of matrices: the requests are cut
+   into column chunks of 8, 4, 2 and 1 columns and each (chunk width, x type)
+   group is one kernel launch over all its tiles; each y is bit-identical to
+   what tc_spmv computes for that request alone */
 int tc_spmv_batch(const tc_request *reqs, int n, void *stream);
 
 /* ---- serving (no reference counterpart): resident matrices, admission by
@@ -222,7 +224,7 @@ int tc_server_destroy(tc_server *s);
 int tc_server_add_matrix(tc_server *s, int M, int N, const TaiChi_new *t, long long nnz, int *id);
 /* TC_ERR_STATE while requests on it are pending or in flight */
 int tc_server_remove_matrix(tc_server *s, int id);
-/* Y[M x k] = A_id * X[N x k] (k in 1..8); X and Y are device buffers owned
+/* Y[M x k] = A_id * X[N x k] (k in 1..64); X and Y are device buffers owned
    by the caller and left untouched by others until the ticket completes */
 int tc_server_submit(tc_server *s, int matrix_id, const void *X, int ldx, int dtype, float *Y,
                      int ldy, int k, unsigned long long *ticket);

@@ -462,7 +462,8 @@ class Matrix:
 
 
 def spmv_batch(items, stream=None):
-    """One persistent launch over [(Matrix, X, Y), ...]; k <= 8 each."""
+    """Batched SpMV over [(Matrix, X, Y), ...] (k <= 64 each): one launch per
+    (column chunk width, x type) group; each Y equals Matrix.spmv alone."""
     reqs = (Request * len(items))()
     for i, (A, X, Y) in enumerate(items):
         k = 1 if X.dim() == 1 else X.shape[1]

@@ -1,6 +1,6 @@
 // Internal interface between the SpMV kernels (spmv.hip) and the serving
-// runtime (serve.hip): descriptor-level enqueue of the persistent batched
-// kernel, so a caller can keep descriptors in its own pinned buffers.
+// runtime (serve.hip): descriptor-level enqueue of the batched kernels, so a
+// caller can keep descriptors in its own pinned buffers.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,11 +9,12 @@
 
 namespace tc {
 
-// bytes of one work descriptor
-size_t batch_desc_size();
-// Fills n descriptors in host_desc (which must stay valid until the copy on
-// stream s has run) and enqueues their copy to dev_desc plus one persistent
-// launch covering all n requests.
+// bytes of descriptor memory (host and device) for up to max_requests
+// requests of k <= 64 columns
+size_t batch_desc_bytes(int max_requests);
+// Fills the descriptors of n requests in host_desc (which must stay valid
+// until the copy on stream s has run) and enqueues their copy to dev_desc
+// plus one launch per (column chunk width, x type) group of the batch.
 int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s);
 
 }  // namespace tc

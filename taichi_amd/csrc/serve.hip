@@ -178,7 +178,7 @@ int submit_at(tc_server *S, int matrix_id, const void *X, int ldx, int dtype, fl
     std::lock_guard<std::mutex> g(S->mu);
     auto it = S->mats.find(matrix_id);
     if (it == S->mats.end()) return tc::fail(TC_ERR_INVALID, "tc_server_submit: unknown matrix %d", matrix_id);
-    if (k < 1 || k > 8 || ldx < k || ldy < k || !X || !Y)
+    if (k < 1 || k > 64 || ldx < k || ldy < k || !X || !Y)
         return tc::fail(TC_ERR_INVALID, "tc_server_submit: bad request (k=%d)", k);
     if (dtype != TC_F32 && dtype != TC_BF16 && dtype != TC_F16) return tc::fail(TC_ERR_INVALID, "bad dtype");
     if (S->stop) return tc::fail(TC_ERR_STATE, "tc_server_submit: server stopping");
@@ -258,7 +258,7 @@ int tc_server_create(const tc_server_config *cfg, tc_server **out)
     S->cfg = *cfg;
     (void)hipGetDevice(&S->device);
     S->slots.resize(cfg->streams);
-    const size_t db = (size_t)cfg->max_batch * tc::batch_desc_size();
+    const size_t db = tc::batch_desc_bytes(cfg->max_batch);
     for (Slot &sl : S->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||

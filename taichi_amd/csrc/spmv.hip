@@ -141,16 +141,18 @@ __device__ __forceinline__ float wave_sum(float v)
            __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
 }
 
-// Compensated (Neumaier) accumulation: s + c carries the running sum with
-// the rounding error of every add kept in c. Dense diagonals add x over
-// whole segments and the remainder subtracts it again at their zero cells,
-// so plain FP32 sums lose accuracy to cancellation; the compensated sum
-// keeps each y within a few ulps of the exact sum, at no cost in a
-// memory-bound kernel. Integer-valued inputs stay exact (c stays 0).
+// Compensated accumulation: s + c carries the running sum with the exact
+// rounding error of every add (Knuth's TwoSum, branch-free) kept in c, the
+// same error term Neumaier's magnitude-ordered form computes. Dense diagonals
+// add x over whole segments and the remainder subtracts it again at their
+// zero cells, so plain FP32 sums lose accuracy to cancellation; the
+// compensated sum keeps each y within a few ulps of the exact sum.
+// Integer-valued inputs stay exact (c stays 0). Needs -ffp-contract=off.
 __device__ __forceinline__ void cadd(float &s, float &c, float x)
 {
     const float t = s + x;
-    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
+    const float z = t - s;
+    c += (s - (t - z)) + (x - z);
     s = t;
 }
 
@@ -248,21 +250,50 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
         }
         __syncthreads();
     }
+    // Offsets outer, pairs inner: each offset is one scalar load and the P
+    // reads behind it are independent. Every pair still adds its terms in
+    // offset order.
+    bool act[P];
+    int base[P];
+    bool inner = true;  // all of this lane's pairs active, all columns inside the square
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const int r = u0 + prow[p];
-        if (!pv[p] || r < ra || r >= rb) continue;
-        const int base = r + shift;
-        float a = acc[p], e = cmp[p];
-        for (int n = sh.ob; n < sh.oe; n++) {
-            const int c = base + cload(A.offs, n);
-            if (c >= clo && c < chi)
-                cadd(a, e, stage ? s_val[(c - w0) * K + pcol[p]] : load_x<XT>(X, (size_t)c * ldx + pcol[p]));
-        }
-        acc[p] = a;
-        cmp[p] = e;
+        act[p] = pv[p] && r >= ra && r < rb;
+        base[p] = r + shift;
+        inner = inner && act[p] && base[p] + sh.omin >= clo && base[p] + sh.omax < chi;
     }
-    if (stage) __syncthreads();
+    if (stage) {
+        int li[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
+        if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+            for (int n = sh.ob; n < sh.oe; n++) {
+                const int o = cload(A.offs, n) * K;
+#pragma unroll
+                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], s_val[li[p] + o]);
+            }
+        } else {
+            for (int n = sh.ob; n < sh.oe; n++) {
+                const int o = cload(A.offs, n);
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    const int c = base[p] + o;
+                    if (act[p] && c >= clo && c < chi) cadd(acc[p], cmp[p], s_val[li[p] + o * K]);
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+        for (int n = sh.ob; n < sh.oe; n++) {
+            const int o = cload(A.offs, n);
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                const int c = base[p] + o;
+                if (act[p] && c >= clo && c < chi) cadd(acc[p], cmp[p], load_x<XT>(X, (size_t)c * ldx + pcol[p]));
+            }
+        }
+    }
 }
 
 template <int K, typename XT, int P>
@@ -387,22 +418,21 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
 // rows form exactly one offset group (their base is one scalar load), and a
 // row's first entry is its left neighbour's end. The tile's entry range is
 // two scalar loads (tiles are 64-row aligned). All loads of a phase are
-// issued before the first use. Same canonical order as run_tile.
+// issued before the first use. Same canonical order as run_tile. Column c of
+// X is X[c * ldx], row r of Y is Y[r * ldy] (1 and 1 for a plain vector).
+// LDS: s_val[capv] | s_lsum[TRW] (LONG) | s_win[TRW] (FAST).
 template <typename XT, int RPT, int OW, bool FAST, bool LONG>
-__global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__restrict__ X, float *__restrict__ Y,
-                                                        int ntiles, int capv)
+__device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
+                                          int ldy, int t, float *smem, int capv)
 {
     constexpr int TRW = kWG * RPT;
-    extern __shared__ float smem[];
     float *s_val = smem;
     float *s_lsum = smem + capv;
     float *s_win = smem + capv + (LONG ? TRW : 0);
     const int tid = threadIdx.x, lane = tid & 63;
-    int tb, te;
-    block_units(ntiles, &tb, &te);
     const uint8_t *ro8 = (const uint8_t *)A.roff;
     const uint16_t *ro16 = (const uint16_t *)A.roff;
-    for (int t = tb; t < te; t++) {
+    {
         const int u0 = t * TRW;
         const int rows = min(TRW, A.M - u0);
         const int ebase = cload(A.gbase, u0 >> 6);
@@ -430,7 +460,7 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
         float xw[RPT];
         if constexpr (FAST) {
 #pragma unroll
-            for (int j = 0; j < RPT; j++) xw[j] = load_x<XT>(X, (size_t)min(max(w0 + prow[j], 0), A.N - 1));
+            for (int j = 0; j < RPT; j++) xw[j] = load_x<XT>(X, (size_t)min(max(w0 + prow[j], 0), A.N - 1) * ldx);
         }
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
@@ -456,7 +486,7 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
             }
             __syncthreads();
         } else {
-            diag_phase<1, XT, RPT>(A, X, 1, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+            diag_phase<1, XT, RPT>(A, X, ldx, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
         }
 
         // a row starts where its left neighbour ends (DPP wave_shr:1); lane 0
@@ -486,7 +516,7 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
                     if (b + u * kWG < ne) {
                         const int col = vv[u] >> 1;
                         if (FAST && (unsigned)(col - w0) < (unsigned)TRW) xv[u] = s_win[col - w0];
-                        else xv[u] = load_x<XT>(X, (size_t)col);
+                        else xv[u] = load_x<XT>(X, (size_t)col * ldx);
                     }
                 }
 #pragma unroll
@@ -507,14 +537,24 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
             __syncthreads();
         }
 
-        if constexpr (LONG) long_phase<1, XT, RPT>(A, X, 1, u0, rows, lq, pv, acc, cmp, s_lsum, TRW);
+        if constexpr (LONG) long_phase<1, XT, RPT>(A, X, ldx, u0, rows, lq, pv, acc, cmp, s_lsum, TRW);
 
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             if (!pv[j]) continue;
-            Y[u0 + prow[j]] = acc[j] + cmp[j];
+            Y[(size_t)(u0 + prow[j]) * ldy] = acc[j] + cmp[j];
         }
     }
+}
+
+template <typename XT, int RPT, int OW, bool FAST, bool LONG>
+__global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__restrict__ X, float *__restrict__ Y,
+                                                        int ntiles, int capv)
+{
+    extern __shared__ float smem[];
+    int tb, te;
+    block_units(ntiles, &tb, &te);
+    for (int t = tb; t < te; t++) rows_tile<XT, RPT, OW, FAST, LONG>(A, X, 1, Y, 1, t, smem, capv);
 }
 
 // LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
@@ -540,76 +580,108 @@ __global__ __launch_bounds__(kWG) void spmv_kernel(MatView A, const XT *__restri
     }
 }
 
-// ---- batched persistent kernel ----
-constexpr int kBatchP = 2;
-constexpr int kBatchLdsDesc = 1024;  // tile starts of up to this many requests go to LDS
+// ---- batched launches over many requests ----
+// A batch is cut into column chunks of K in {8, 4, 2, 1} columns and the
+// chunks are grouped by (K, x type); each group is one launch of a kernel
+// specialised for it, so no launch carries the registers of the others. The
+// tiles of a group are numbered consecutively, chunk by chunk; a workgroup
+// finds its chunk by binary search over the group's tile starts (staged in
+// LDS) and reads the chunk's descriptor with scalar loads. Column chunks
+// change nothing in the result: the per-(row, column) order is independent
+// of K.
+constexpr int kBatchP = 2;       // K > 1: run_tile with P = 2 (512/K rows a tile)
+constexpr int kBatchRPT = 2;     // K = 1: rows_tile with 512 rows a tile
+constexpr int kBatchCapv = 4096; // staged values per pass
+constexpr int kBatchLdsDesc = 512;
 
 struct WorkDesc {
     MatView A;
     const void *X;
     float *Y;
-    int ldx, ldy, K, dtype;
-    int unit_begin, pad0;
+    int ldx, ldy;
+    int unit_begin;
+    int fast;  // one shape with one offset: the rows tile's FAST path
 };
 
-template <int K, typename XT>
-__device__ __forceinline__ void batch_unit(const WorkDesc &w, int local, float *s_val, int capv, int *s_start,
-                                           float *s_lsum)
+__device__ __forceinline__ WorkDesc load_desc(const WorkDesc *D, int d)
 {
-    constexpr int TRW = (kWG * kBatchP) / K;
-    run_tile<K, XT, kBatchP>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, local * TRW, s_val, capv, s_start, s_lsum);
+    constexpr int NW = sizeof(WorkDesc) / 4;
+    union {
+        WorkDesc w;
+        int u[NW];
+    } t;
+    const int *p = (const int *)(D + d);
+#pragma unroll
+    for (int i = 0; i < NW; i++) t.u[i] = cload(p, i);
+    return t.w;
+}
+
+// Runs f(desc, local_tile) for every tile of this workgroup's range of the
+// group's `total` tiles. UB[i] is chunk i's first tile (ascending).
+template <typename F>
+__device__ __forceinline__ void batch_loop(const WorkDesc *__restrict__ D, const int *__restrict__ UB, int nd,
+                                           int total, int *s_ub, F &&f)
+{
+    int ub, ue;
+    block_units(total, &ub, &ue);
+    if (ub >= ue) return;
+    const bool in_lds = nd <= kBatchLdsDesc;
+    if (in_lds) {
+        for (int i = threadIdx.x; i < nd; i += kWG) s_ub[i] = UB[i];
+        __syncthreads();
+    }
+    int lo = 0, hi = nd - 1;  // last chunk with UB <= ub
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((in_lds ? s_ub[mid] : cload(UB, mid)) <= ub) lo = mid;
+        else hi = mid - 1;
+    }
+    int d = __builtin_amdgcn_readfirstlane(lo);
+    for (int u = ub; u < ue; u++) {
+        while (d + 1 < nd && (in_lds ? s_ub[d + 1] : cload(UB, d + 1)) <= u) d++;
+        d = __builtin_amdgcn_readfirstlane(d);
+        const WorkDesc w = load_desc(D, d);
+        f(w, u - w.unit_begin);
+        __syncthreads();
+    }
 }
 
 template <typename XT>
-__device__ __forceinline__ void batch_dispatch_k(const WorkDesc &w, int local, float *s_val, int capv, int *s_start,
-                                                 float *s_lsum)
-{
-    switch (w.K) {
-    case 1: batch_unit<1, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    case 2: batch_unit<2, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    case 3: batch_unit<3, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    case 4: batch_unit<4, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    case 5: batch_unit<5, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    case 6: batch_unit<6, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    case 7: batch_unit<7, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    default: batch_unit<8, XT>(w, local, s_val, capv, s_start, s_lsum); break;
-    }
-}
-
-__global__ __launch_bounds__(kWG) void batch_kernel(const WorkDesc *__restrict__ D, int ndesc, int total_units,
-                                                    int capv)
+__global__ __launch_bounds__(kWG) void batch_rows_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
+                                                         int nd, int total, int capv)
 {
     extern __shared__ float smem[];
+    __shared__ int s_ub[kBatchLdsDesc];
+    batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
+        const XT *X = (const XT *)w.X;
+        const bool lng = w.A.L > 0;
+        if (w.A.ow == 1) {
+            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 1, true, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, kBatchRPT, 1, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, kBatchRPT, 1, false, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, kBatchRPT, 1, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+        } else {
+            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 2, true, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, kBatchRPT, 2, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, kBatchRPT, 2, false, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, kBatchRPT, 2, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+        }
+    });
+}
+
+template <int K, typename XT>
+__global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
+                                                         int nd, int total, int capv)
+{
+    constexpr int TRW = (kWG * kBatchP) / K;
+    extern __shared__ float smem[];
+    __shared__ int s_ub[kBatchLdsDesc];
     float *s_val = smem;
     float *s_lsum = s_val + capv;
     int *s_start = (int *)(s_lsum + kWG * kBatchP);
-    __shared__ int s_ub[kBatchLdsDesc];
-    int ub, ue;
-    block_units(total_units, &ub, &ue);
-    if (ub >= ue) return;
-    // last descriptor with unit_begin <= ub: searched in LDS (one coalesced
-    // load of the tile starts) when the batch is small enough
-    const bool in_lds = ndesc <= kBatchLdsDesc;
-    if (in_lds) {
-        for (int i = threadIdx.x; i < ndesc; i += kWG) s_ub[i] = D[i].unit_begin;
-        __syncthreads();
-    }
-    int lo = 0, hi = ndesc - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if ((in_lds ? s_ub[mid] : D[mid].unit_begin) <= ub) lo = mid;
-        else hi = mid - 1;
-    }
-    int d = lo;
-    for (int u = ub; u < ue; u++) {
-        while (d + 1 < ndesc && D[d + 1].unit_begin <= u) d++;
-        const WorkDesc w = D[d];
-        const int local = u - w.unit_begin;
-        if (w.dtype == TC_BF16) batch_dispatch_k<bf16_t>(w, local, s_val, capv, s_start, s_lsum);
-        else if (w.dtype == TC_F16) batch_dispatch_k<f16_t>(w, local, s_val, capv, s_start, s_lsum);
-        else batch_dispatch_k<float>(w, local, s_val, capv, s_start, s_lsum);
-        __syncthreads();
-    }
+    batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
+        run_tile<K, XT, kBatchP>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t * TRW, s_val, capv, s_start, s_lsum);
+    });
 }
 
 }  // namespace
@@ -673,19 +745,27 @@ int pick_k(int k)
     return 1;
 }
 
-tc_config default_config(int K)
+// Tiles of 512 rows (k = 1) or 512 (row, column) pairs (k > 1) and a
+// 4096-value stage: the best or near-best setting measured on R-MAT and
+// banded matrices. Matrices whose remainder averages more than 8 entries a
+// row (banded with many zero cells) use 256-row tiles at k = 1, so a tile's
+// entries pass through the stage in about two rounds.
+tc_config default_config(int K, long long remainder, int M)
 {
     tc_config c{};
-    c.rows_per_tile = K == 1 ? 2048 : (kWG * 2) / K;  // K=1: 8 rows per thread; else P = 2
+    const bool heavy = M > 0 && remainder > 8ll * M;
+    c.rows_per_tile = K == 1 ? (heavy ? 256 : 512) : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
     c.flags = 0;
     return c;
 }
 
+tc_config default_config(const tc_matrix *A, int K) { return default_config(K, A->st.remainder, A->v.M); }
+
 bool rows_per_tile_ok(int K, int r)
 {
-    if (K == 1) return r == 512 || r == 1024 || r == 2048 || r == 4096;
+    if (K == 1) return r == 256 || r == 512 || r == 1024 || r == 2048 || r == 4096;
     for (int P : {1, 2, 4})
         if ((kWG * P) / K == r) return true;
     return false;
@@ -728,6 +808,7 @@ template <typename XT>
 int launch_rows(const MatView &A, const void *X, float *Y, const tc_config &c, int capv, hipStream_t s)
 {
     switch (c.rows_per_tile) {
+    case 256: return launch_rows_t<XT, 1>(A, X, Y, capv, c.grid, s);
     case 512: return launch_rows_t<XT, 2>(A, X, Y, capv, c.grid, s);
     case 1024: return launch_rows_t<XT, 4>(A, X, Y, capv, c.grid, s);
     case 4096: return launch_rows_t<XT, 16>(A, X, Y, capv, c.grid, s);
@@ -790,7 +871,7 @@ tc_config get_cfg(tc_matrix *A, int K, int dtype)
 {
     std::lock_guard<std::mutex> g(A->mu);
     auto it = A->cfg.find({K, dtype});
-    return it != A->cfg.end() ? it->second : default_config(K);
+    return it != A->cfg.end() ? it->second : default_config(A, K);
 }
 
 // one launch of K columns (K a supported count) with an explicit config
@@ -823,43 +904,105 @@ int check_spmv_args(const tc_matrix *A, const void *X, int ldx, int dtype, const
 
 namespace tc {
 
-size_t batch_desc_size() { return sizeof(WorkDesc); }
+// chunks of one request of k columns: 8s, then 4, 2, 1 as needed
+int chunk_count(int k) { return k / 8 + ((k & 4) != 0) + ((k & 2) != 0) + (k & 1); }
 
-// Fills n descriptors in host_desc (pinned or pageable memory that outlives
-// the copy) and enqueues their copy to dev_desc and one persistent launch on
-// s. Each request's y equals what tc_spmv computes for it alone.
+constexpr int kMaxBatchK = 64;
+
+size_t batch_desc_bytes(int max_requests)
+{
+    const size_t chunks = (size_t)max_requests * chunk_count(kMaxBatchK - 1);
+    return chunks * sizeof(WorkDesc) + chunks * sizeof(int) + 16;
+}
+
+template <typename XT>
+void launch_batch_class(int K, const WorkDesc *D, const int *UB, int nd, int units, hipStream_t s)
+{
+    const int grid = units;  // one workgroup per tile: the dispatcher balances chunks of any size
+    if (K == 1) {
+        constexpr int TRW = kWG * kBatchRPT;
+        const size_t lds = ((size_t)kBatchCapv + 2 * TRW) * 4;
+        batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+        return;
+    }
+    const size_t lds = lds_bytes(kBatchCapv, kBatchP, (kWG * kBatchP) / K);
+    if (K == 2) batch_tile_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+    else if (K == 4) batch_tile_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+    else batch_tile_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+}
+
+// Fills the chunk descriptors of n requests into host_desc (at least
+// batch_desc_bytes(n) bytes, valid until the copy on s has run), copies them
+// to dev_desc and enqueues one launch per (K, x type) group present. Each
+// request's y equals what tc_spmv computes for it alone.
 int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s)
 {
-    WorkDesc *D = (WorkDesc *)host_desc;
-    long long units = 0;
+    static const int kChunkK[4] = {1, 2, 4, 8};
+    int cls_count[12] = {0};
+    int nchunks = 0;
     for (int i = 0; i < n; i++) {
         const tc_request &r = reqs[i];
         int rc = check_spmv_args(r.A, r.X, r.ldx, r.dtype, r.Y, r.ldy, r.k);
         if (rc != TC_OK) return rc;
-        if (r.k > 8) return tc::fail(TC_ERR_UNSUPPORTED, "batch: k=%d > 8", r.k);
-        WorkDesc &w = D[i];
-        w.A = r.A->v;
-        w.X = r.X;
-        w.Y = r.Y;
-        w.ldx = r.ldx;
-        w.ldy = r.ldy;
-        w.K = r.k;
-        w.dtype = r.dtype;
-        w.unit_begin = (int)units;
-        w.pad0 = 0;
-        const int trw = (kWG * kBatchP) / r.k;
-        units += (r.A->v.M + trw - 1) / trw;
+        if (r.k > kMaxBatchK) return tc::fail(TC_ERR_UNSUPPORTED, "batch: k=%d > %d", r.k, kMaxBatchK);
+        const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
+        for (int ki = 0; ki < 4; ki++) {
+            const int K = kChunkK[ki];
+            const int c = K == 8 ? r.k / 8 : (r.k & K) != 0;
+            cls_count[ki * 3 + dt] += c;
+        }
+        nchunks += chunk_count(r.k);
     }
-    if (units > INT32_MAX) return tc::fail(TC_ERR_UNSUPPORTED, "batch: too many tiles");
-    if (units == 0) return TC_OK;
-    HIPCHECK(hipMemcpyAsync(dev_desc, D, (size_t)n * sizeof(WorkDesc), hipMemcpyHostToDevice, s));
-    const int capv = 4096;
-    // one workgroup per tile: the hardware scheduler balances requests of
-    // very different sizes
-    const int grid = (int)units;
-    const size_t lds = lds_bytes(capv, kBatchP, kWG * kBatchP);
-    batch_kernel<<<grid, kWG, lds, s>>>((const WorkDesc *)dev_desc, n, (int)units, capv);
-    HIPCHECK(hipGetLastError());
+    if (nchunks == 0) return TC_OK;
+    int cls_begin[13] = {0};
+    for (int c = 0; c < 12; c++) cls_begin[c + 1] = cls_begin[c] + cls_count[c];
+    WorkDesc *D = (WorkDesc *)host_desc;
+    const size_t ub_off = ((size_t)nchunks * sizeof(WorkDesc) + 15) & ~(size_t)15;
+    int *UB = (int *)((char *)host_desc + ub_off);
+    int fill[12];
+    long long units[12] = {0};
+    for (int c = 0; c < 12; c++) fill[c] = cls_begin[c];
+    // chunks in request order within each group, widest chunk first
+    for (int i = 0; i < n; i++) {
+        const tc_request &r = reqs[i];
+        const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
+        const MatView &v = r.A->v;
+        const bool fast = v.S == 1 && v.sh0.oe - v.sh0.ob == 1;
+        int c0 = 0;
+        for (int ki = 3; ki >= 0; ki--) {
+            const int K = kChunkK[ki];
+            while (r.k - c0 >= K && (K == 8 || ((r.k - c0) & K))) {
+                const int cl = ki * 3 + dt;
+                WorkDesc &w = D[fill[cl]];
+                w.A = v;
+                w.X = (const char *)r.X + (size_t)c0 * elem_size(r.dtype);
+                w.Y = r.Y + c0;
+                w.ldx = r.ldx;
+                w.ldy = r.ldy;
+                w.fast = fast ? 1 : 0;
+                const int trw = K == 1 ? kWG * kBatchRPT : (kWG * kBatchP) / K;
+                w.unit_begin = (int)units[cl];
+                UB[fill[cl]] = w.unit_begin;
+                units[cl] += (v.M + trw - 1) / trw;
+                fill[cl]++;
+                c0 += K;
+                if (K != 8) break;
+            }
+        }
+    }
+    for (int c = 0; c < 12; c++)
+        if (units[c] > INT32_MAX) return tc::fail(TC_ERR_UNSUPPORTED, "batch: too many tiles");
+    HIPCHECK(hipMemcpyAsync(dev_desc, host_desc, ub_off + (size_t)nchunks * sizeof(int), hipMemcpyHostToDevice, s));
+    const WorkDesc *dD = (const WorkDesc *)dev_desc;
+    const int *dUB = (const int *)((const char *)dev_desc + ub_off);
+    for (int c = 0; c < 12; c++) {
+        if (units[c] == 0) continue;
+        const int K = kChunkK[c / 3], nd = cls_count[c], b = cls_begin[c];
+        if (c % 3 == 0) launch_batch_class<float>(K, dD + b, dUB + b, nd, (int)units[c], s);
+        else if (c % 3 == 1) launch_batch_class<bf16_t>(K, dD + b, dUB + b, nd, (int)units[c], s);
+        else launch_batch_class<f16_t>(K, dD + b, dUB + b, nd, (int)units[c], s);
+        HIPCHECK(hipGetLastError());
+    }
     return TC_OK;
 }
 
@@ -1142,7 +1285,7 @@ int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg)
     if (!A || !cfg || k < 1) return tc::fail(TC_ERR_INVALID, "tc_matrix_set_config: bad argument");
     const int K = pick_k(k);
     tc_config c = *cfg;
-    const tc_config d = default_config(K);
+    const tc_config d = default_config(A, K);
     if (c.rows_per_tile <= 0) c.rows_per_tile = d.rows_per_tile;
     if (c.lds_values <= 0) c.lds_values = d.lds_values;
     if (!rows_per_tile_ok(K, c.rows_per_tile)) return tc::fail(TC_ERR_INVALID, "rows_per_tile %d not available for k=%d", c.rows_per_tile, K);
@@ -1174,7 +1317,7 @@ int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int l
     const int ncu = cu_count();
     std::vector<tc_config> cands;
     std::vector<int> tiles;
-    if (K == 1) tiles = {512, 1024, 2048, 4096};
+    if (K == 1) tiles = {256, 512, 1024, 2048, 4096};
     else
         for (int P : {1, 2, 4}) tiles.push_back((kWG * P) / K);
     for (int rpt : tiles)
@@ -1185,7 +1328,7 @@ int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int l
                     tc_config c{rpt, capv, grid, flags};
                     cands.push_back(c);
                 }
-    tc_config bc = default_config(K);
+    tc_config bc = default_config(A, K);
     float bt = 1e30f;
     const int reps = 20;
     for (const tc_config &c : cands) {
@@ -1221,7 +1364,7 @@ int tc_spmv_batch(const tc_request *reqs, int n, void *stream)
     if (n < 0 || (n > 0 && !reqs)) return tc::fail(TC_ERR_INVALID, "tc_spmv_batch: bad argument");
     if (n == 0) return TC_OK;
     hipStream_t s = (hipStream_t)stream;
-    std::vector<char> host((size_t)n * tc::batch_desc_size());
+    std::vector<char> host(tc::batch_desc_bytes(n));
     void *dd = nullptr;
     HIPCHECK(hipMallocAsync(&dd, host.size(), s));
     int rc = tc::batch_enqueue(reqs, n, host.data(), dd, s);

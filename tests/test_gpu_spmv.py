@@ -165,6 +165,42 @@ def test_batch_matches_alone(dev):
         assert torch.equal(Yb, Ya)
 
 
+@pytest.mark.parametrize("ks", [(3, 7, 13, 20), (64, 1, 6, 11)])
+def test_batch_column_chunks(dev, ks):
+    """Requests wider than a chunk are cut into 8/4/2/1-column pieces, x types
+    mixed, strided column views, a one-offset shape (single diagonal) and long
+    rows: every Y equals the request run alone."""
+    mats = []
+    n0 = 9000
+    rp, col = T.gen_banded(n0, 1, 1.0, 7)                      # one shape, one offset
+    mats.append((n0, host_format(n0, n0, rp, col)))
+    rp, col = T.gen_banded(n0 + 123, 9, 0.7, 8)                # one shape, 9 offsets
+    mats.append((n0 + 123, host_format(n0 + 123, n0 + 123, rp, col)))
+    n2, _, rp, col = rmat(14)
+    mats.append((n2, host_format(n2, n2, rp, col)))
+    n3 = 5000                                                  # long rows
+    rows = [np.arange(0, n3, 3, dtype=np.int32) if r % 97 == 0 else np.array([r], np.int32) for r in range(n3)]
+    rp = np.zeros(n3 + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    mats.append((n3, host_format(n3, n3, rp, col)))
+    items, alone = [], []
+    for i, ((n, fmt), k) in enumerate(zip(mats, ks)):
+        dt = [torch.float32, torch.bfloat16, torch.float16, torch.float32][i]
+        big = torch.from_numpy(T.gen_vector(n * (k + 3), 2, 300 + i).reshape(n, k + 3)).to(dev).to(dt)
+        X = big[:, 2:2 + k]                                    # ldx = k + 3
+        A = T.Matrix(fmt)
+        Yb = torch.full((n, k + 1), float("nan"), device=dev)[:, :k]
+        Ya = torch.full((n, k), float("nan"), device=dev)
+        A.spmv(X, Ya)
+        items.append((A, X, Yb))
+        alone.append(Ya)
+    T.spmv_batch(items)
+    torch.cuda.synchronize()
+    for (A, X, Yb), Ya in zip(items, alone):
+        assert torch.equal(Yb, Ya)
+
+
 def test_graph_replay_matches_eager(dev):
     M, N, rp, col = rmat(16)
     fmt = host_format(M, N, rp, col)
