@@ -168,15 +168,17 @@ int tc_spmv(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy,
 
 /* launch configuration (no reference counterpart; the reference's TPV tier,
    TaiChi.cu:1390-1394, plays this role): rows per workgroup tile (k = 1:
-   512, 1024, 2048 or 4096 = 2..16 rows per lane; k > 1: 256*P/k for P in
-   1, 2, 4), LDS value capacity, persistent grid size (0 = one workgroup per
-   tile). Results do not depend on the configuration. */
+   256, 512, 1024, 2048 or 4096 = 1..16 rows per lane; k > 1: 256*P/k for P
+   in 1, 2, 4), LDS value capacity, persistent grid size (0 = one workgroup
+   per tile), kernel choice. Results do not depend on the configuration. */
 typedef struct tc_config {
     int rows_per_tile;   /* power of two, 1..2048; 0 = default for k */
     int lds_values;      /* LDS staging capacity in floats (512..8192); 0 = default */
     int grid;            /* 0 = one workgroup per tile */
-    int flags;           /* bit 0: non-temporal loads of the column stream and
-                            stores of y (k = 1 kernel) */
+    int flags;           /* bit 0: reserved (ignored). bit 1: at k = 1 use the
+                            one-row-per-lane kernel, not quad rows (quad rows
+                            apply to 512/1024-row tiles of matrices with at
+                            most one shape of one offset) */
 } tc_config;
 int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg);
 int tc_matrix_get_config(const tc_matrix *A, int k, int dtype, tc_config *cfg);

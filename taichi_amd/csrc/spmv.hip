@@ -185,7 +185,7 @@ __device__ __forceinline__ void block_units(int total, int *b, int *e)
 // s_val[j*K + c]. The loads of each batch (U items per lane) are all issued
 // before the first gather waits on them; lanes past the end re-read a valid
 // item and do not store.
-template <int K, typename XT, bool NT, int U = kStageU>
+template <int K, typename XT, int U = kStageU>
 __device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne, const XT *__restrict__ X, int ldx,
                                               float *s_val)
 {
@@ -203,7 +203,7 @@ __device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne
                 const int idx = min(b + u * kWG + tid, n - 1);
                 const int j = idx / K;
                 cc[u] = idx - j * K;
-                v[u] = NT ? __builtin_nontemporal_load(cp + j) : cp[j];
+                v[u] = cp[j];
             }
         }
         float xv[U];
@@ -387,7 +387,7 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
     const int cape = capv / K;
     for (int e0 = 0; e0 < E; e0 += cape) {
         const int ne = min(cape, E - e0);
-        stage_entries<K, XT, false>(A.cols + ebase + e0, ne, X, ldx, s_val);
+        stage_entries<K, XT>(A.cols + ebase + e0, ne, X, ldx, s_val);
         __syncthreads();
 #pragma unroll
         for (int p = 0; p < P; p++) {
@@ -555,6 +555,208 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
     int tb, te;
     block_units(ntiles, &tb, &te);
     for (int t = tb; t < te; t++) rows_tile<XT, RPT, OW, FAST, LONG>(A, X, 1, Y, 1, t, smem, capv);
+}
+
+// ---- single right-hand side, quad rows: at most one shape with one offset ----
+// Lane l of a workgroup owns the Q consecutive rows u0 + Q*l + i (i < Q) of a
+// tile of 256*Q rows. Every row-indexed stream -- local offsets, the x window
+// along the diagonal, y -- moves Q elements per lane in one load or store, and
+// the tile's remainder entries are read as 16-byte-aligned int4 per lane
+// (starting at the 4-entry boundary below the tile's first entry). Against
+// one row per lane this cuts the vector memory instructions per row about
+// threefold; on R-MAT the one-row-per-lane kernel is bound by their issue
+// (texture addresser busy most of the kernel), not by bytes. Same canonical
+// per-row order as every other kernel. Requires X and Y aligned to Q
+// elements (checked on the host).
+// LDS: s_val[capv] (also the long-row sums afterwards) | s_win[TRW] (DIAG).
+template <typename XT, int Q>
+__device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
+{
+    if constexpr (sizeof(XT) == 4) {
+        if constexpr (Q == 4) {
+            const float4 v = *(const float4 *)p;
+            o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w;
+        } else {
+            const float2 v = *(const float2 *)p;
+            o[0] = v.x, o[1] = v.y;
+        }
+    } else {
+        uint32_t w[Q / 2];
+        if constexpr (Q == 4) {
+            const uint2 v = *(const uint2 *)p;
+            w[0] = v.x, w[1] = v.y;
+        } else {
+            w[0] = *(const uint32_t *)p;
+        }
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            const uint16_t h = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+            if constexpr (sizeof(XT) == 2 && __is_same(XT, bf16_t)) o[i] = __uint_as_float((uint32_t)h << 16);
+            else o[i] = (float)__builtin_bit_cast(_Float16, h);
+        }
+    }
+}
+
+template <int OW, int Q>
+__device__ __forceinline__ void load_ends(const void *roff, int r0, int (&e)[Q])
+{
+    if constexpr (OW == 1) {
+        const uint32_t w = Q == 4 ? *(const uint32_t *)((const uint8_t *)roff + r0)
+                                  : (uint32_t) * (const uint16_t *)((const uint8_t *)roff + r0);
+#pragma unroll
+        for (int i = 0; i < Q; i++) e[i] = (int)((w >> (8 * i)) & 0xFF);
+    } else {
+        uint32_t w[Q / 2];
+        if constexpr (Q == 4) {
+            const uint2 v = *(const uint2 *)((const uint16_t *)roff + r0);
+            w[0] = v.x, w[1] = v.y;
+        } else {
+            w[0] = *(const uint32_t *)((const uint16_t *)roff + r0);
+        }
+#pragma unroll
+        for (int i = 0; i < Q; i++) e[i] = (int)((w[i >> 1] >> (16 * (i & 1))) & 0xFFFF);
+    }
+}
+
+template <typename XT, int Q, int OW, bool DIAG, bool LONG>
+__device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict__ X, float *__restrict__ Y, int t,
+                                          float *smem, int capv)
+{
+    constexpr int TRW = kWG * Q;
+    float *s_val = smem;
+    float *s_win = smem + capv;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int u0 = t * TRW;
+    const int r0 = u0 + Q * tid;  // this lane's first row
+    const bool any = r0 < A.M;
+    const bool full = r0 + Q <= A.M;
+
+    // the tile's entries [ebase, eend), streamed from the 4-aligned lb
+    const int ebase = cload(A.gbase, u0 >> 6);
+    const int eend = cload(A.gbase, u0 + TRW >= A.M ? A.ngroups : (u0 + TRW) >> 6);
+    const int lb = ebase & ~3;
+    const int EA = eend - lb;
+
+    // issue order: row ends, the x window, the first int4 of the stream
+    int endv[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) endv[i] = 0;
+    if (any) load_ends<OW, Q>(A.roff, r0, endv);
+    const int G = A.ngroups;
+    const int gw = __builtin_amdgcn_readfirstlane((u0 + Q * 64 * (tid >> 6)) >> 6);  // the wavefront's first group
+    int gb;
+    if constexpr (Q == 4) {
+        const int g0 = cload(A.gbase, min(gw, G)), g1 = cload(A.gbase, min(gw + 1, G));
+        const int g2 = cload(A.gbase, min(gw + 2, G)), g3 = cload(A.gbase, min(gw + 3, G));
+        gb = lane < 16 ? g0 : lane < 32 ? g1 : lane < 48 ? g2 : g3;
+    } else {
+        const int g0 = cload(A.gbase, min(gw, G)), g1 = cload(A.gbase, min(gw + 1, G));
+        gb = lane < 32 ? g0 : g1;
+    }
+    const int w0 = DIAG ? u0 + A.sh0.ys - A.sh0.xs + A.sh0.omin : 0;  // x index of local row 0's diagonal term
+    float xw[Q];
+    if constexpr (DIAG) {
+        const int c0 = w0 + Q * tid;
+        if ((w0 & (Q - 1)) == 0 && c0 >= 0 && c0 + Q <= A.N) {
+            load_q<XT, Q>(X + c0, xw);
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; i++) xw[i] = load_x<XT>(X, (size_t)min(max(c0 + i, 0), A.N - 1));
+        }
+    }
+    int4 cv = make_int4(1, 1, 1, 1);
+    if (4 * tid < min(EA, capv)) cv = *(const int4 *)(A.cols + lb + 4 * tid);
+
+    float acc[Q], cmp[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) acc[i] = 0.0f, cmp[i] = 0.0f;
+    if constexpr (DIAG) {
+        const ShapeDev &sh = A.sh0;
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            const int r = r0 + i, c = w0 + Q * tid + i;
+            if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs))
+                cadd(acc[i], cmp[i], xw[i]);
+        }
+        if constexpr (Q == 4) *(float4 *)(s_win + Q * tid) = make_float4(xw[0], xw[1], xw[2], xw[3]);
+        else *(float2 *)(s_win + Q * tid) = make_float2(xw[0], xw[1]);
+        __syncthreads();
+    }
+
+    // row i of this lane: entries [rs[i], re[i]) counted from lb; a row
+    // starts where its left neighbour ends (DPP wave_shr:1), a 64-row group
+    // at 0
+    const int prev = __builtin_amdgcn_update_dpp(0, endv[Q - 1], 0x138, 0xF, 0xF, false);
+    int rs[Q], re[Q];
+    rs[0] = gb - lb + (((Q * lane) & 63) == 0 ? 0 : prev);
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+        re[i] = r0 + i < A.M ? gb - lb + endv[i] : rs[i];
+        if (i + 1 < Q) rs[i + 1] = re[i];
+    }
+
+    for (int e0 = 0; e0 < EA; e0 += capv) {
+        const int ne = min(capv, EA - e0);
+        for (int b = 0; b < ne; b += kWG * 4) {
+            const int q = b + 4 * tid;
+            int4 v = make_int4(1, 1, 1, 1);
+            if (e0 == 0 && b == 0) v = cv;
+            else if (q < ne) v = *(const int4 *)(A.cols + lb + e0 + q);
+            const int vv[4] = {v.x, v.y, v.z, v.w};
+            float xv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int col = vv[k] >> 1;
+                xv[k] = 0.0f;
+                if (DIAG && (unsigned)(col - w0) < (unsigned)TRW) xv[k] = s_win[col - w0];
+                else if (q + k < ne) xv[k] = load_x<XT>(X, (size_t)col);
+            }
+            if (q < ne)
+                *(float4 *)(s_val + q) = make_float4((vv[0] & 1) ? xv[0] : -xv[0], (vv[1] & 1) ? xv[1] : -xv[1],
+                                                     (vv[2] & 1) ? xv[2] : -xv[2], (vv[3] & 1) ? xv[3] : -xv[3]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            const int lo = max(rs[i], e0), hi = min(re[i], e0 + ne);
+            float a = acc[i], e = cmp[i];
+            for (int k = lo; k < hi; k++) cadd(a, e, s_val[k - e0]);
+            acc[i] = a;
+            cmp[i] = e;
+        }
+        __syncthreads();
+    }
+
+    if constexpr (LONG) {
+        int prow[Q], lq[Q];
+        bool pv[Q];
+#pragma unroll
+        for (int i = 0; i < Q; i++) prow[i] = Q * tid + i, lq[i] = prow[i], pv[i] = r0 + i < A.M;
+        long_phase<1, XT, Q>(A, X, 1, u0, min(TRW, A.M - u0), lq, pv, acc, cmp, s_val, TRW);
+    }
+
+    if (full) {
+        if constexpr (Q == 4)
+            *(float4 *)(Y + r0) = make_float4(acc[0] + cmp[0], acc[1] + cmp[1], acc[2] + cmp[2], acc[3] + cmp[3]);
+        else *(float2 *)(Y + r0) = make_float2(acc[0] + cmp[0], acc[1] + cmp[1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; i++)
+            if (r0 + i < A.M) Y[r0 + i] = acc[i] + cmp[i];
+    }
+}
+
+template <typename XT, int Q, int OW, bool DIAG, bool LONG>
+__global__ __launch_bounds__(kWG) void spmv_quad_kernel(MatView A, const XT *__restrict__ X, float *__restrict__ Y,
+                                                        int ntiles, int capv)
+{
+    extern __shared__ float smem[];
+    int tb, te;
+    block_units(ntiles, &tb, &te);
+    for (int t = tb; t < te; t++) {
+        quad_tile<XT, Q, OW, DIAG, LONG>(A, X, Y, t, smem, capv);
+        if (DIAG && t + 1 < te) __syncthreads();  // s_win is rewritten by the next tile
+    }
 }
 
 // LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
@@ -750,6 +952,8 @@ int pick_k(int k)
 // banded matrices. Matrices whose remainder averages more than 8 entries a
 // row (banded with many zero cells) use 256-row tiles at k = 1, so a tile's
 // entries pass through the stage in about two rounds.
+constexpr int kQuadOff = 2;  // config flag: use the one-row-per-lane kernel at k = 1
+
 tc_config default_config(int K, long long remainder, int M)
 {
     tc_config c{};
@@ -757,7 +961,7 @@ tc_config default_config(int K, long long remainder, int M)
     c.rows_per_tile = K == 1 ? (heavy ? 256 : 512) : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
-    c.flags = 0;
+    c.flags = K == 1 ? kQuadOff : 0;  // quad rows measured no faster on R-MAT; the autotuner tries both
     return c;
 }
 
@@ -814,6 +1018,57 @@ int launch_rows(const MatView &A, const void *X, float *Y, const tc_config &c, i
     case 4096: return launch_rows_t<XT, 16>(A, X, Y, capv, c.grid, s);
     default: return launch_rows_t<XT, 8>(A, X, Y, capv, c.grid, s);
     }
+}
+
+template <typename XT, int Q, int OW, bool DIAG, bool LONG>
+void launch_quad_k(const MatView &A, const void *X, float *Y, int ntiles, int capv, int G, size_t lds, hipStream_t s)
+{
+    spmv_quad_kernel<XT, Q, OW, DIAG, LONG><<<G, kWG, lds, s>>>(A, (const XT *)X, Y, ntiles, capv);
+}
+
+template <typename XT, int Q>
+int launch_quad_t(const MatView &A, const void *X, float *Y, int capv, int grid, hipStream_t s)
+{
+    constexpr int TRW = kWG * Q;
+    const int ntiles = (A.M + TRW - 1) / TRW;
+    if (ntiles == 0) return TC_OK;
+    const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
+    const bool diag = A.S == 1;
+    const bool lng = A.L > 0;
+    const size_t lds = ((size_t)capv + (diag ? TRW : 0)) * 4;
+    if (A.ow == 1) {
+        if (diag) lng ? launch_quad_k<XT, Q, 1, true, true>(A, X, Y, ntiles, capv, G, lds, s)
+                      : launch_quad_k<XT, Q, 1, true, false>(A, X, Y, ntiles, capv, G, lds, s);
+        else lng ? launch_quad_k<XT, Q, 1, false, true>(A, X, Y, ntiles, capv, G, lds, s)
+                 : launch_quad_k<XT, Q, 1, false, false>(A, X, Y, ntiles, capv, G, lds, s);
+    } else {
+        if (diag) lng ? launch_quad_k<XT, Q, 2, true, true>(A, X, Y, ntiles, capv, G, lds, s)
+                      : launch_quad_k<XT, Q, 2, true, false>(A, X, Y, ntiles, capv, G, lds, s);
+        else lng ? launch_quad_k<XT, Q, 2, false, true>(A, X, Y, ntiles, capv, G, lds, s)
+                 : launch_quad_k<XT, Q, 2, false, false>(A, X, Y, ntiles, capv, G, lds, s);
+    }
+    HIPCHECK(hipGetLastError());
+    return TC_OK;
+}
+
+// Quad-row kernel for k = 1 when it applies: no shape or one shape with one
+// offset, tiles of 512 or 1024 rows, X and Y aligned to the rows per lane,
+// and not switched off by config flag bit 1 (kQuadOff).
+
+bool quad_applies(const MatView &A, const void *X, const float *Y, int esize, const tc_config &c, int capv)
+{
+    if (c.flags & kQuadOff) return false;
+    if (!(A.S == 0 || (A.S == 1 && A.sh0.oe - A.sh0.ob == 1))) return false;
+    const int Q = c.rows_per_tile == 512 ? 2 : c.rows_per_tile == 1024 ? 4 : 0;
+    if (Q == 0 || capv % 4 != 0 || capv < kWG * Q) return false;
+    return ((uintptr_t)X % ((uintptr_t)Q * esize)) == 0 && ((uintptr_t)Y % ((uintptr_t)Q * 4)) == 0;
+}
+
+template <typename XT>
+int launch_quad(const MatView &A, const void *X, float *Y, const tc_config &c, int capv, hipStream_t s)
+{
+    if (c.rows_per_tile == 1024) return launch_quad_t<XT, 4>(A, X, Y, capv, c.grid, s);
+    return launch_quad_t<XT, 2>(A, X, Y, capv, c.grid, s);
 }
 
 // P from a rows_per_tile request for column count K (1, 2 or 4)
@@ -879,6 +1134,11 @@ int launch_cfg(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ld
                hipStream_t s)
 {
     const int capv = std::max(c.lds_values, 64 * K);
+    if (K == 1 && ldx == 1 && ldy == 1 && quad_applies(A->v, X, Y, elem_size(dtype), c, capv)) {
+        if (dtype == TC_BF16) return launch_quad<bf16_t>(A->v, X, Y, c, capv, s);
+        if (dtype == TC_F16) return launch_quad<f16_t>(A->v, X, Y, c, capv, s);
+        return launch_quad<float>(A->v, X, Y, c, capv, s);
+    }
     if (K == 1 && ldx == 1 && ldy == 1 && ((uintptr_t)Y & 15) == 0) {
         if (dtype == TC_BF16) return launch_rows<bf16_t>(A->v, X, Y, c, capv, s);
         if (dtype == TC_F16) return launch_rows<f16_t>(A->v, X, Y, c, capv, s);
@@ -1323,7 +1583,7 @@ int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int l
     for (int rpt : tiles)
         for (int capv : {2048, 4096, 8192})
             for (int grid : {0, 4 * ncu, 8 * ncu})
-                for (int flags : {0, 1}) {
+                for (int flags : {0, kQuadOff}) {
                     if (capv < 64 * K || (flags && K != 1)) continue;
                     tc_config c{rpt, capv, grid, flags};
                     cands.push_back(c);

@@ -115,6 +115,88 @@ def test_long_rows_wavefront(dev):
     check_float(y, rp, col, xr.astype(np.float64))
 
 
+def _offdiag_band(n, off, extra, seed):
+    """One full off-diagonal (row r, column r + off) plus `extra` uniform
+    entries per row: a single shape whose x window is not 4-aligned."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for r in range(n):
+        c = set(rng.integers(0, n, extra).tolist())
+        if 0 <= r + off < n:
+            c.add(r + off)
+        rows.append(np.array(sorted(c), np.int32))
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    return rp, np.concatenate(rows).astype(np.int32)
+
+
+def _local_mix(n, extra, spread, long_every, long_len, long_spread, seed):
+    """Main diagonal, `extra` entries per row within +-spread of it, and every
+    long_every-th row with long_len entries within +-long_spread (long rows)."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for r in range(n):
+        if long_every and r % long_every == 0:
+            m, w = long_len, long_spread
+        else:
+            m, w = extra, spread
+        c = set(np.clip(r + rng.integers(-w, w + 1, m), 0, n - 1).tolist())
+        c.add(r)
+        rows.append(np.array(sorted(c), np.int32))
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    return rp, np.concatenate(rows).astype(np.int32)
+
+
+@pytest.mark.parametrize("case", ["rmat17", "offdiag", "long", "plain_csr", "wide_groups"])
+def test_k1_kernels_agree(case, dev):
+    """Every k = 1 kernel (quad rows at 512 and 1024-row tiles, one row per
+    lane at 256..4096) and stage size gives the oracle's y: bit-exact for
+    integer x, and bit-identical to each other for random x."""
+    if case == "rmat17":
+        M, N, rp, col = rmat(17)
+    elif case == "offdiag":
+        M = N = 70001
+        rp, col = _offdiag_band(M, 3, 1, 21)
+    elif case == "long":
+        M = N = 50003
+        rp, col = _local_mix(M, 1, 40, 509, 200, 20000, 22)
+    elif case == "plain_csr":
+        M = N = 3000  # thumbnail blocks of 5 rows: no shape survives, S = 0
+        rp, col = T.gen_rmat(M, 12, 30000, 0.25, 0.25, 0.25, 23)
+    else:  # 64-row groups above 255 entries: 16-bit local offsets
+        M = N = 120000
+        rp, col = _local_mix(M, 6, 300, 0, 0, 0, 24)
+    fmt = host_format(M, N, rp, col)
+    A = T.Matrix(fmt)
+    xi = torch.from_numpy(T.gen_vector(N, 1, 31)).to(dev)
+    xr = torch.from_numpy(T.gen_vector(N, 2, 32)).to(dev)
+    want = O.csr_spmv_f32(rp, col, xi.cpu().numpy())
+    ref = None
+    for rows in (256, 512, 1024, 2048, 4096):
+        for capv in (1024, 4096):
+            for flags in (0, 2):
+                A.set_config(1, T.F32, rows, capv, 0, flags)
+                y = torch.full((M,), float("nan"), device=dev)
+                A.spmv(xi, y)
+                torch.cuda.synchronize()
+                assert np.array_equal(y.cpu().numpy(), want), (rows, capv, flags)
+                A.spmv(xr, y)
+                torch.cuda.synchronize()
+                if ref is None:
+                    ref = y.clone()
+                    check_float(y.cpu().numpy(), rp, col, xr.cpu().numpy().astype(np.float64))
+                assert torch.equal(y, ref), (rows, capv, flags)
+    if case == "offdiag":
+        assert fmt.dia_shapes == 1 and fmt.num_negs == 1
+    if case == "long":
+        assert A.stats()["long_rows"] > 0 and fmt.dia_shapes == 1 and fmt.num_negs == 1
+    if case == "plain_csr":
+        assert fmt.dia_shapes == 0
+    if case == "wide_groups":
+        assert A.stats()["offset_bytes"] == 2 and fmt.dia_shapes == 1 and fmt.num_negs == 1
+
+
 @pytest.mark.parametrize("k", [1, 3, 4, 16, 64])
 def test_multi_rhs(k, dev):
     M, N, rp, col = rmat(16)
