@@ -290,6 +290,20 @@ def main():
         cold.append(e0.elapsed_time(e1))
     ms_cold = float(np.median(cold))
     del scrub
+
+    # host buffers (the taichi_SpMV boundary): x uploaded and y downloaded
+    # through pinned host memory around every SpMV, PCIe included
+    xh = torch.from_numpy(x).pin_memory()
+    yh = torch.empty(n, dtype=torch.float32).pin_memory()
+    host = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        xd.copy_(xh, non_blocking=True)
+        A.spmv(xd, yd)
+        yh.copy_(yd, non_blocking=True)
+        torch.cuda.synchronize()
+        host.append(time.perf_counter() - t0)
+    ms_host = float(np.median(host)) * 1e3
     if args.calibrate_copy:
         src = torch.ones(1 << 26, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -327,6 +341,8 @@ def main():
         "eager_ms_per_step": round(ms_eager, 5),
         "eager_gnnz_per_s": round(nnz / (ms_eager * 1e-3) / 1e9, 3),
         "cold_ms_per_step": round(ms_cold, 5),
+        "host_buffers_ms_per_step": round(ms_host, 5),
+        "host_buffers_gnnz_per_s": round(nnz / (ms_host * 1e-3) / 1e9, 3),
         "max_rel_err_vs_fp64": err,
         "wall_s_timed": round(wall, 4),
     }
