@@ -759,6 +759,118 @@ __global__ __launch_bounds__(kWG) void spmv_quad_kernel(MatView A, const XT *__r
     }
 }
 
+// ---- k > 1: wide rows ----
+// A row belongs to a group of LPR = K/CPL lanes; lane q of the group owns the
+// CPL = min(K, 4) consecutive columns q*CPL.., so each x row segment the row
+// touches (a diagonal term or a remainder entry) is one CPL-wide vector load
+// per lane and y leaves as one vector store per lane. Consecutive groups own
+// consecutive rows (y is written contiguously when ldy = K). Rows are
+// independent: no LDS staging; only long rows (one wavefront each, the fixed
+// butterfly) meet in LDS. Same canonical per-(row, column) order as every
+// other kernel. Requires X, Y, ldx and ldy aligned to CPL elements.
+// LDS (LONG): s_lsum[TRW * K].
+template <int K, typename XT, int RI, bool LONG>
+__global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
+                                                        float *__restrict__ Y, int ldy, int ntiles)
+{
+    constexpr int CPL = K < 4 ? K : 4;
+    constexpr int LPR = K / CPL;
+    constexpr int GPW = kWG / LPR;  // row groups per workgroup
+    constexpr int TRW = GPW * RI;   // rows per tile
+    constexpr int P = CPL * RI;     // (row, column) pairs per lane
+    constexpr int U = 4;            // remainder entries in flight per lane
+    extern __shared__ float s_lsum[];
+    const int tid = threadIdx.x;
+    const int g = tid / LPR, c0 = (tid % LPR) * CPL;
+    const uint8_t *ro8 = (const uint8_t *)A.roff;
+    const uint16_t *ro16 = (const uint16_t *)A.roff;
+    int tb, te;
+    block_units(ntiles, &tb, &te);
+    for (int t = tb; t < te; t++) {
+        const int u0 = t * TRW;
+        float acc[P], cmp[P];
+        int lq[P];
+        bool pv[P];
+        int rs[RI], re[RI];
+#pragma unroll
+        for (int i = 0; i < RI; i++) {
+            const int r = u0 + GPW * i + g;
+            const bool valid = r < A.M;
+            const int rr = valid ? r : A.M - 1;
+            const int gb = A.gbase[rr >> 6];
+            const int prev = (rr & 63) ? (A.ow == 1 ? (int)ro8[rr - 1] : (int)ro16[rr - 1]) : 0;
+            const int end = A.ow == 1 ? (int)ro8[rr] : (int)ro16[rr];
+            rs[i] = gb + prev;
+            re[i] = valid ? gb + end : rs[i];
+#pragma unroll
+            for (int m = 0; m < CPL; m++) {
+                acc[i * CPL + m] = 0.0f;
+                cmp[i * CPL + m] = 0.0f;
+                lq[i * CPL + m] = (GPW * i + g) * K + c0 + m;
+                pv[i * CPL + m] = valid;
+            }
+        }
+
+        // diagonal terms: shapes in order, offsets in order; the rows' loads
+        // of one offset are issued together
+        for (int s = 0; s < A.S; s++) {
+            const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
+            if (u0 + TRW <= sh.xs || u0 >= sh.xe) continue;
+            const int side = sh.xe - sh.xs;
+            for (int n = sh.ob; n < sh.oe; n++) {
+                const int o = cload(A.offs, n);
+                float xv[RI][CPL];
+                bool in[RI];
+#pragma unroll
+                for (int i = 0; i < RI; i++) {
+                    const int r = u0 + GPW * i + g;
+                    const int c = r - sh.xs + sh.ys + o;
+                    in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
+                    if (in[i]) load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < RI; i++)
+                    if (in[i])
+#pragma unroll
+                        for (int m = 0; m < CPL; m++) cadd(acc[i * CPL + m], cmp[i * CPL + m], xv[i][m]);
+            }
+        }
+
+        // remainder entries, U in flight, added in stored order
+#pragma unroll
+        for (int i = 0; i < RI; i++) {
+            for (int j0 = rs[i]; j0 < re[i]; j0 += U) {
+                int v[U];
+                float xv[U][CPL];
+#pragma unroll
+                for (int u = 0; u < U; u++) v[u] = j0 + u < re[i] ? A.cols[j0 + u] : 1;
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j0 + u < re[i]) load_q<XT, CPL>(X + (size_t)(v[u] >> 1) * ldx + c0, xv[u]);
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j0 + u < re[i])
+#pragma unroll
+                        for (int m = 0; m < CPL; m++)
+                            cadd(acc[i * CPL + m], cmp[i * CPL + m], (v[u] & 1) ? xv[u][m] : -xv[u][m]);
+            }
+        }
+
+        if constexpr (LONG) long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, cmp, s_lsum, TRW * K);
+
+#pragma unroll
+        for (int i = 0; i < RI; i++) {
+            const int r = u0 + GPW * i + g;
+            if (r >= A.M) continue;
+            float *yp = Y + (size_t)r * ldy + c0;
+            if constexpr (CPL == 4)
+                *(float4 *)yp = make_float4(acc[i * 4] + cmp[i * 4], acc[i * 4 + 1] + cmp[i * 4 + 1],
+                                            acc[i * 4 + 2] + cmp[i * 4 + 2], acc[i * 4 + 3] + cmp[i * 4 + 3]);
+            else *(float2 *)yp = make_float2(acc[i * 2] + cmp[i * 2], acc[i * 2 + 1] + cmp[i * 2 + 1]);
+        }
+    }
+}
+
 // LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
 __host__ __device__ constexpr size_t lds_bytes(int capv, int P, int trw)
 {
@@ -952,13 +1064,23 @@ int pick_k(int k)
 // banded matrices. Matrices whose remainder averages more than 8 entries a
 // row (banded with many zero cells) use 256-row tiles at k = 1, so a tile's
 // entries pass through the stage in about two rounds.
+// k > 1: the wide-rows kernel serves K in {2, 4, 8, 16, 32, 64} unless
+// config flag bit 2 (kStagedK) asks for the staged tile kernel, which also
+// serves every other K.
+constexpr int kStagedK = 4;
+
+bool wide_k(int K) { return K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64; }
+
+// rows of a wide-rows tile with RI rows per lane group
+int wide_rows(int K, int RI) { return (kWG / (K / std::min(K, 4))) * RI; }
+
 constexpr int kQuadOff = 2;  // config flag: use the one-row-per-lane kernel at k = 1
 
 tc_config default_config(int K, long long remainder, int M)
 {
     tc_config c{};
     const bool heavy = M > 0 && remainder > 8ll * M;
-    c.rows_per_tile = K == 1 ? (heavy ? 256 : 512) : (kWG * 2) / K;
+    c.rows_per_tile = K == 1 ? (heavy ? 256 : 512) : wide_k(K) ? wide_rows(K, 2) : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
     c.flags = K == 1 ? kQuadOff : 0;  // quad rows measured no faster on R-MAT; the autotuner tries both
@@ -967,9 +1089,10 @@ tc_config default_config(int K, long long remainder, int M)
 
 tc_config default_config(const tc_matrix *A, int K) { return default_config(K, A->st.remainder, A->v.M); }
 
-bool rows_per_tile_ok(int K, int r)
+bool rows_per_tile_ok(int K, int r, int flags)
 {
     if (K == 1) return r == 256 || r == 512 || r == 1024 || r == 2048 || r == 4096;
+    if (wide_k(K) && !(flags & kStagedK)) return r == wide_rows(K, 1) || r == wide_rows(K, 2);
     for (int P : {1, 2, 4})
         if ((kWG * P) / K == r) return true;
     return false;
@@ -1129,6 +1252,52 @@ tc_config get_cfg(tc_matrix *A, int K, int dtype)
     return it != A->cfg.end() ? it->second : default_config(A, K);
 }
 
+template <int K, typename XT, int RI>
+int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int grid, hipStream_t s)
+{
+    constexpr int TRW = (kWG / (K / (K < 4 ? K : 4))) * RI;
+    const int ntiles = (A.M + TRW - 1) / TRW;
+    if (ntiles == 0) return TC_OK;
+    const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
+    if (A.L > 0)
+        spmv_wide_kernel<K, XT, RI, true><<<G, kWG, (size_t)TRW * K * 4, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
+    else
+        spmv_wide_kernel<K, XT, RI, false><<<G, kWG, 0, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
+    HIPCHECK(hipGetLastError());
+    return TC_OK;
+}
+
+template <int K, typename XT>
+int launch_wide_k(const MatView &A, const void *X, int ldx, float *Y, int ldy, const tc_config &c, hipStream_t s)
+{
+    if (c.rows_per_tile == wide_rows(K, 1)) return launch_wide_t<K, XT, 1>(A, X, ldx, Y, ldy, c.grid, s);
+    return launch_wide_t<K, XT, 2>(A, X, ldx, Y, ldy, c.grid, s);
+}
+
+template <typename XT>
+int launch_wide(const MatView &A, const void *X, int ldx, float *Y, int ldy, int K, const tc_config &c,
+                hipStream_t s)
+{
+    switch (K) {
+    case 2: return launch_wide_k<2, XT>(A, X, ldx, Y, ldy, c, s);
+    case 4: return launch_wide_k<4, XT>(A, X, ldx, Y, ldy, c, s);
+    case 8: return launch_wide_k<8, XT>(A, X, ldx, Y, ldy, c, s);
+    case 16: return launch_wide_k<16, XT>(A, X, ldx, Y, ldy, c, s);
+    case 32: return launch_wide_k<32, XT>(A, X, ldx, Y, ldy, c, s);
+    default: return launch_wide_k<64, XT>(A, X, ldx, Y, ldy, c, s);
+    }
+}
+
+// wide rows apply when X and Y rows split into aligned CPL-element vectors
+bool wide_applies(int K, const void *X, int ldx, const float *Y, int ldy, int esize, const tc_config &c)
+{
+    if (!wide_k(K) || (c.flags & kStagedK)) return false;
+    if (c.rows_per_tile != wide_rows(K, 1) && c.rows_per_tile != wide_rows(K, 2)) return false;
+    const int cpl = std::min(K, 4);
+    return ldx % cpl == 0 && ldy % cpl == 0 && (uintptr_t)X % ((uintptr_t)cpl * esize) == 0 &&
+           (uintptr_t)Y % ((uintptr_t)cpl * 4) == 0;
+}
+
 // one launch of K columns (K a supported count) with an explicit config
 int launch_cfg(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int K, const tc_config &c,
                hipStream_t s)
@@ -1144,7 +1313,12 @@ int launch_cfg(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ld
         if (dtype == TC_F16) return launch_rows<f16_t>(A->v, X, Y, c, capv, s);
         return launch_rows<float>(A->v, X, Y, c, capv, s);
     }
-    // generic tile kernel: K=1 configs name the rows kernel's tile; use P = 2
+    if (K > 1 && wide_applies(K, X, ldx, Y, ldy, elem_size(dtype), c)) {
+        if (dtype == TC_BF16) return launch_wide<bf16_t>(A->v, X, ldx, Y, ldy, K, c, s);
+        if (dtype == TC_F16) return launch_wide<f16_t>(A->v, X, ldx, Y, ldy, K, c, s);
+        return launch_wide<float>(A->v, X, ldx, Y, ldy, K, c, s);
+    }
+    // staged tile kernel: K=1 configs name the rows kernel's tile; use P = 2
     const int P = K == 1 ? 2 : p_for(K, c.rows_per_tile);
     if (dtype == TC_BF16) return launch_k<bf16_t>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
     if (dtype == TC_F16) return launch_k<f16_t>(A->v, X, ldx, Y, ldy, K, P, capv, c.grid, s);
@@ -1548,7 +1722,8 @@ int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg)
     const tc_config d = default_config(A, K);
     if (c.rows_per_tile <= 0) c.rows_per_tile = d.rows_per_tile;
     if (c.lds_values <= 0) c.lds_values = d.lds_values;
-    if (!rows_per_tile_ok(K, c.rows_per_tile)) return tc::fail(TC_ERR_INVALID, "rows_per_tile %d not available for k=%d", c.rows_per_tile, K);
+    if (!rows_per_tile_ok(K, c.rows_per_tile, c.flags))
+        return tc::fail(TC_ERR_INVALID, "rows_per_tile %d not available for k=%d flags=%d", c.rows_per_tile, K, c.flags);
     if (c.lds_values < 512 || c.lds_values > 8192)
         return tc::fail(TC_ERR_INVALID, "lds_values %d outside [512, 8192]", c.lds_values);
     if (c.grid < 0) return tc::fail(TC_ERR_INVALID, "grid < 0");
@@ -1577,17 +1752,25 @@ int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int l
     const int ncu = cu_count();
     std::vector<tc_config> cands;
     std::vector<int> tiles;
-    if (K == 1) tiles = {256, 512, 1024, 2048, 4096};
-    else
-        for (int P : {1, 2, 4}) tiles.push_back((kWG * P) / K);
-    for (int rpt : tiles)
+    // (rows_per_tile, flags) pairs: both k = 1 kernels; wide and staged for k > 1
+    std::vector<std::pair<int, int>> tf;
+    if (K == 1) {
+        for (int r : {256, 512, 1024, 2048, 4096})
+            for (int f : {0, kQuadOff}) tf.push_back({r, f});
+    } else {
+        if (wide_k(K))
+            for (int RI : {1, 2}) tf.push_back({wide_rows(K, RI), 0});
+        for (int P : {1, 2, 4}) tf.push_back({(kWG * P) / K, kStagedK});
+    }
+    for (auto [rpt, flags] : tf)
         for (int capv : {2048, 4096, 8192})
-            for (int grid : {0, 4 * ncu, 8 * ncu})
-                for (int flags : {0, kQuadOff}) {
-                    if (capv < 64 * K || (flags && K != 1)) continue;
-                    tc_config c{rpt, capv, grid, flags};
-                    cands.push_back(c);
-                }
+            for (int grid : {0, 4 * ncu, 8 * ncu}) {
+                if (capv < 64 * K) continue;
+                if (K == 1 && (flags & kQuadOff) == 0 && rpt != 512 && rpt != 1024) continue;
+                if (K > 1 && !(flags & kStagedK) && capv != 4096) continue;  // the wide kernel stages nothing
+                tc_config c{rpt, capv, grid, flags};
+                cands.push_back(c);
+            }
     tc_config bc = default_config(A, K);
     float bt = 1e30f;
     const int reps = 20;

@@ -213,6 +213,49 @@ def test_multi_rhs(k, dev):
     assert np.array_equal(Y[:, 0], y0)
 
 
+@pytest.mark.parametrize("case", ["banded", "long"])
+def test_k_columns_wide_equals_staged(case, dev):
+    """For every k the wide-rows kernel serves (2..64), each tile size of it
+    and the staged kernel give identical Y; integer X is bit-exact against the
+    CPU CSR per column; BF16 X is exact to its quantised FP64 product."""
+    if case == "banded":  # one shape, 17 offsets, 16-bit local offsets
+        M = N = 30011
+        rp, col = T.gen_banded(M, 17, 0.7, 41)
+    else:  # a diagonal shape plus long rows
+        M = N = 50003
+        rp, col = _local_mix(M, 1, 40, 509, 200, 20000, 42)
+    fmt = host_format(M, N, rp, col)
+    A = T.Matrix(fmt)
+    if case == "long":
+        assert A.stats()["long_rows"] > 0
+    for k in (2, 4, 8, 16, 32, 64):
+        Xi = T.gen_vector(N * k, 1, 50 + k).reshape(N, k)
+        Xd = torch.from_numpy(Xi).to(dev)
+        outs = []
+        cpl = min(k, 4)
+        for rows, flags in [(256 * cpl // k, 0), (512 * cpl // k, 0), (512 // k, 4)]:
+            A.set_config(k, T.F32, rows, 4096, 0, flags)
+            Y = torch.full((M, k), float("nan"), device=dev)
+            A.spmv(Xd, Y)
+            torch.cuda.synchronize()
+            outs.append(Y.cpu().numpy())
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0]), k
+        for c in (0, k - 1):
+            assert np.array_equal(outs[0][:, c], O.csr_spmv_f32(rp, col, np.ascontiguousarray(Xi[:, c]))), (k, c)
+    k = 8
+    Xr = T.gen_vector(N * k, 2, 77).reshape(N, k)
+    Xb = torch.from_numpy(Xr).to(dev).to(torch.bfloat16)
+    Y = torch.empty(M, k, device=dev)
+    A.spmv(Xb, Y)
+    torch.cuda.synchronize()
+    Xq = Xb.float().cpu().numpy().astype(np.float64)
+    Y64 = O.csr_spmm_f64(rp, col, Xq)
+    den = np.maximum(np.abs(Y64), O.csr_spmm_f64(rp, col, np.abs(Xq)))
+    den[den == 0] = 1
+    assert float(np.max(np.abs(Y.cpu().numpy() - Y64) / den)) <= 1e-6
+
+
 @pytest.mark.parametrize("dt", [torch.bfloat16, torch.float16])
 def test_half_storage(dt, dev):
     M, N, rp, col = rmat(16)
