@@ -770,8 +770,8 @@ __global__ __launch_bounds__(kWG) void spmv_quad_kernel(MatView A, const XT *__r
 // other kernel. Requires X, Y, ldx and ldy aligned to CPL elements.
 // LDS (LONG): s_lsum[TRW * K].
 template <int K, typename XT, int RI, bool LONG>
-__global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
-                                                        float *__restrict__ Y, int ldy, int ntiles)
+__device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
+                                          int ldy, int t, float *s_lsum)
 {
     constexpr int CPL = K < 4 ? K : 4;
     constexpr int LPR = K / CPL;
@@ -779,14 +779,11 @@ __global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__r
     constexpr int TRW = GPW * RI;   // rows per tile
     constexpr int P = CPL * RI;     // (row, column) pairs per lane
     constexpr int U = 4;            // remainder entries in flight per lane
-    extern __shared__ float s_lsum[];
     const int tid = threadIdx.x;
     const int g = tid / LPR, c0 = (tid % LPR) * CPL;
     const uint8_t *ro8 = (const uint8_t *)A.roff;
     const uint16_t *ro16 = (const uint16_t *)A.roff;
-    int tb, te;
-    block_units(ntiles, &tb, &te);
-    for (int t = tb; t < te; t++) {
+    {
         const int u0 = t * TRW;
         float acc[P], cmp[P];
         int lq[P];
@@ -869,6 +866,16 @@ __global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__r
             else *(float2 *)yp = make_float2(acc[i * 2] + cmp[i * 2], acc[i * 2 + 1] + cmp[i * 2 + 1]);
         }
     }
+}
+
+template <int K, typename XT, int RI, bool LONG>
+__global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
+                                                        float *__restrict__ Y, int ldy, int ntiles)
+{
+    extern __shared__ float s_lsum[];
+    int tb, te;
+    block_units(ntiles, &tb, &te);
+    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG>(A, X, ldx, Y, ldy, t, s_lsum);
 }
 
 // LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
@@ -995,6 +1002,21 @@ __global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restr
     int *s_start = (int *)(s_lsum + kWG * kBatchP);
     batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
         run_tile<K, XT, kBatchP>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t * TRW, s_val, capv, s_start, s_lsum);
+    });
+}
+
+// K > 1 chunks whose X and Y rows split into aligned vectors: wide-rows tiles
+constexpr int kBatchRI = 2;
+
+template <int K, typename XT>
+__global__ __launch_bounds__(kWG) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
+                                                         int nd, int total)
+{
+    extern __shared__ float s_lsum[];
+    __shared__ int s_ub[kBatchLdsDesc];
+    batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
+        if (w.A.L > 0) wide_tile<K, XT, kBatchRI, true>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, s_lsum);
+        else wide_tile<K, XT, kBatchRI, false>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, s_lsum);
     });
 }
 
@@ -1349,30 +1371,72 @@ size_t batch_desc_bytes(int max_requests)
     return chunks * sizeof(WorkDesc) + chunks * sizeof(int) + 16;
 }
 
+// Batch classes: (kernel, chunk width, x type). Kernel 0: rows tiles (K = 1);
+// 1: wide-rows tiles (K > 1, aligned); 2: staged tiles (K > 1, otherwise).
+constexpr int kBatchClasses = 3 * 4 * 3;
+inline int batch_class(int kern, int ki, int dt) { return (kern * 4 + ki) * 3 + dt; }
+
+int batch_tile_rows(int kern, int K)
+{
+    if (kern == 0) return kWG * kBatchRPT;
+    if (kern == 1) return wide_rows(K, kBatchRI);
+    return (kWG * kBatchP) / K;
+}
+
 template <typename XT>
-void launch_batch_class(int K, const WorkDesc *D, const int *UB, int nd, int units, hipStream_t s)
+void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int nd, int units, hipStream_t s)
 {
     const int grid = units;  // one workgroup per tile: the dispatcher balances chunks of any size
-    if (K == 1) {
+    if (kern == 0) {
         constexpr int TRW = kWG * kBatchRPT;
         const size_t lds = ((size_t)kBatchCapv + 2 * TRW) * 4;
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
-        return;
+    } else if (kern == 1) {
+        const size_t lds = (size_t)wide_rows(K, kBatchRI) * K * 4;
+        if (K == 2) batch_wide_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+        else if (K == 4) batch_wide_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+        else batch_wide_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+    } else {
+        const size_t lds = lds_bytes(kBatchCapv, kBatchP, (kWG * kBatchP) / K);
+        if (K == 2) batch_tile_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+        else if (K == 4) batch_tile_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+        else batch_tile_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     }
-    const size_t lds = lds_bytes(kBatchCapv, kBatchP, (kWG * kBatchP) / K);
-    if (K == 2) batch_tile_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
-    else if (K == 4) batch_tile_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
-    else batch_tile_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+}
+
+// chunk c0..c0+K of request r: its batch class
+int chunk_class(const tc_request &r, int ki, int K, int c0, int dt)
+{
+    if (K == 1) return batch_class(0, ki, dt);
+    const int es = elem_size(r.dtype), cpl = std::min(K, 4);
+    const uintptr_t xa = (uintptr_t)r.X + (uintptr_t)c0 * es, ya = (uintptr_t)(r.Y + c0);
+    const bool aligned = r.ldx % cpl == 0 && r.ldy % cpl == 0 && xa % ((uintptr_t)cpl * es) == 0 &&
+                         ya % ((uintptr_t)cpl * 4) == 0;
+    return batch_class(aligned ? 1 : 2, ki, dt);
 }
 
 // Fills the chunk descriptors of n requests into host_desc (at least
 // batch_desc_bytes(n) bytes, valid until the copy on s has run), copies them
-// to dev_desc and enqueues one launch per (K, x type) group present. Each
+// to dev_desc and enqueues one launch per batch class present. Each
 // request's y equals what tc_spmv computes for it alone.
 int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s)
 {
     static const int kChunkK[4] = {1, 2, 4, 8};
-    int cls_count[12] = {0};
+    // chunks of a request: 8s, then 4, 2, 1 as k's bits say; calls
+    // f(ki, K, c0) for each, widest first
+    auto chunks = [](int k, auto &&f) {
+        int c0 = 0;
+        while (k - c0 >= 8) {
+            f(3, 8, c0);
+            c0 += 8;
+        }
+        for (int ki = 2; ki >= 0; ki--)
+            if ((k - c0) & (1 << ki)) {
+                f(ki, 1 << ki, c0);
+                c0 += 1 << ki;
+            }
+    };
+    int cls_count[kBatchClasses] = {0};
     int nchunks = 0;
     for (int i = 0; i < n; i++) {
         const tc_request &r = reqs[i];
@@ -1380,61 +1444,53 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
         if (rc != TC_OK) return rc;
         if (r.k > kMaxBatchK) return tc::fail(TC_ERR_UNSUPPORTED, "batch: k=%d > %d", r.k, kMaxBatchK);
         const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
-        for (int ki = 0; ki < 4; ki++) {
-            const int K = kChunkK[ki];
-            const int c = K == 8 ? r.k / 8 : (r.k & K) != 0;
-            cls_count[ki * 3 + dt] += c;
-        }
-        nchunks += chunk_count(r.k);
+        chunks(r.k, [&](int ki, int K, int c0) {
+            cls_count[chunk_class(r, ki, K, c0, dt)]++;
+            nchunks++;
+        });
     }
     if (nchunks == 0) return TC_OK;
-    int cls_begin[13] = {0};
-    for (int c = 0; c < 12; c++) cls_begin[c + 1] = cls_begin[c] + cls_count[c];
+    int cls_begin[kBatchClasses + 1] = {0};
+    for (int c = 0; c < kBatchClasses; c++) cls_begin[c + 1] = cls_begin[c] + cls_count[c];
     WorkDesc *D = (WorkDesc *)host_desc;
     const size_t ub_off = ((size_t)nchunks * sizeof(WorkDesc) + 15) & ~(size_t)15;
     int *UB = (int *)((char *)host_desc + ub_off);
-    int fill[12];
-    long long units[12] = {0};
-    for (int c = 0; c < 12; c++) fill[c] = cls_begin[c];
-    // chunks in request order within each group, widest chunk first
+    int fill[kBatchClasses];
+    long long units[kBatchClasses] = {0};
+    for (int c = 0; c < kBatchClasses; c++) fill[c] = cls_begin[c];
+    // chunks in request order within each class
     for (int i = 0; i < n; i++) {
         const tc_request &r = reqs[i];
         const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
         const MatView &v = r.A->v;
         const bool fast = v.S == 1 && v.sh0.oe - v.sh0.ob == 1;
-        int c0 = 0;
-        for (int ki = 3; ki >= 0; ki--) {
-            const int K = kChunkK[ki];
-            while (r.k - c0 >= K && (K == 8 || ((r.k - c0) & K))) {
-                const int cl = ki * 3 + dt;
-                WorkDesc &w = D[fill[cl]];
-                w.A = v;
-                w.X = (const char *)r.X + (size_t)c0 * elem_size(r.dtype);
-                w.Y = r.Y + c0;
-                w.ldx = r.ldx;
-                w.ldy = r.ldy;
-                w.fast = fast ? 1 : 0;
-                const int trw = K == 1 ? kWG * kBatchRPT : (kWG * kBatchP) / K;
-                w.unit_begin = (int)units[cl];
-                UB[fill[cl]] = w.unit_begin;
-                units[cl] += (v.M + trw - 1) / trw;
-                fill[cl]++;
-                c0 += K;
-                if (K != 8) break;
-            }
-        }
+        chunks(r.k, [&](int ki, int K, int c0) {
+            const int cl = chunk_class(r, ki, K, c0, dt);
+            WorkDesc &w = D[fill[cl]];
+            w.A = v;
+            w.X = (const char *)r.X + (size_t)c0 * elem_size(r.dtype);
+            w.Y = r.Y + c0;
+            w.ldx = r.ldx;
+            w.ldy = r.ldy;
+            w.fast = fast ? 1 : 0;
+            const int trw = batch_tile_rows(cl / 12, K);
+            w.unit_begin = (int)units[cl];
+            UB[fill[cl]] = w.unit_begin;
+            units[cl] += (v.M + trw - 1) / trw;
+            fill[cl]++;
+        });
     }
-    for (int c = 0; c < 12; c++)
+    for (int c = 0; c < kBatchClasses; c++)
         if (units[c] > INT32_MAX) return tc::fail(TC_ERR_UNSUPPORTED, "batch: too many tiles");
     HIPCHECK(hipMemcpyAsync(dev_desc, host_desc, ub_off + (size_t)nchunks * sizeof(int), hipMemcpyHostToDevice, s));
     const WorkDesc *dD = (const WorkDesc *)dev_desc;
     const int *dUB = (const int *)((const char *)dev_desc + ub_off);
-    for (int c = 0; c < 12; c++) {
+    for (int c = 0; c < kBatchClasses; c++) {
         if (units[c] == 0) continue;
-        const int K = kChunkK[c / 3], nd = cls_count[c], b = cls_begin[c];
-        if (c % 3 == 0) launch_batch_class<float>(K, dD + b, dUB + b, nd, (int)units[c], s);
-        else if (c % 3 == 1) launch_batch_class<bf16_t>(K, dD + b, dUB + b, nd, (int)units[c], s);
-        else launch_batch_class<f16_t>(K, dD + b, dUB + b, nd, (int)units[c], s);
+        const int kern = c / 12, K = kChunkK[(c / 3) % 4], dt = c % 3, nd = cls_count[c], b = cls_begin[c];
+        if (dt == 0) launch_batch_class<float>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
+        else if (dt == 1) launch_batch_class<bf16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
+        else launch_batch_class<f16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
         HIPCHECK(hipGetLastError());
     }
     return TC_OK;
