@@ -768,10 +768,12 @@ __global__ __launch_bounds__(kWG) void spmv_quad_kernel(MatView A, const XT *__r
 // independent: no LDS staging; only long rows (one wavefront each, the fixed
 // butterfly) meet in LDS. Same canonical per-(row, column) order as every
 // other kernel. Requires X, Y, ldx and ldy aligned to CPL elements.
-// LDS (LONG): s_lsum[TRW * K].
+// LDS: the x window [(TRW + kWideHalo) * K], reused for the long-row sums.
+constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
+
 template <int K, typename XT, int RI, bool LONG>
 __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
-                                          int ldy, int t, float *s_lsum)
+                                          int ldy, int t, float *smem)
 {
     constexpr int CPL = K < 4 ? K : 4;
     constexpr int LPR = K / CPL;
@@ -809,11 +811,62 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
         }
 
         // diagonal terms: shapes in order, offsets in order; the rows' loads
-        // of one offset are issued together
+        // of one offset are issued together. A shape with two or more
+        // offsets first stages the tile's x window (rows [w0, w1), K columns)
+        // in LDS, so each x row segment is loaded once per tile, not once
+        // per offset.
         for (int s = 0; s < A.S; s++) {
             const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
             if (u0 + TRW <= sh.xs || u0 >= sh.xe) continue;
             const int side = sh.xe - sh.xs;
+            const int shift = sh.ys - sh.xs;
+            const int ra = max(u0, sh.xs), rb = min(min(u0 + TRW, sh.xe), A.M);
+            const int w0 = max(sh.ys, ra + shift + sh.omin);
+            const int w1 = min(sh.ys + side, rb - 1 + shift + sh.omax + 1);
+            if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
+                const int nv = (w1 - w0) * LPR;  // CPL-wide vectors of the window
+                for (int b = 0; b < nv; b += kWG * 4) {
+                    float tv[4][CPL];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int v = min(b + u * kWG + tid, nv - 1);
+                        load_q<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL, tv[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int v = b + u * kWG + tid;
+                        if (v < nv) {
+                            float *d = smem + (v / LPR) * K + (v % LPR) * CPL;
+                            if constexpr (CPL == 4) *(float4 *)d = make_float4(tv[u][0], tv[u][1], tv[u][2], tv[u][3]);
+                            else *(float2 *)d = make_float2(tv[u][0], tv[u][1]);
+                        }
+                    }
+                }
+                __syncthreads();
+                for (int n = sh.ob; n < sh.oe; n++) {
+                    const int o = cload(A.offs, n);
+#pragma unroll
+                    for (int i = 0; i < RI; i++) {
+                        const int r = u0 + GPW * i + g;
+                        const int c = r + shift + o;
+                        if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
+                            const float *sp = smem + (c - w0) * K + c0;
+                            float xv[CPL];
+                            if constexpr (CPL == 4) {
+                                const float4 q = *(const float4 *)sp;
+                                xv[0] = q.x, xv[1] = q.y, xv[2] = q.z, xv[3] = q.w;
+                            } else {
+                                const float2 q = *(const float2 *)sp;
+                                xv[0] = q.x, xv[1] = q.y;
+                            }
+#pragma unroll
+                            for (int m = 0; m < CPL; m++) cadd(acc[i * CPL + m], cmp[i * CPL + m], xv[m]);
+                        }
+                    }
+                }
+                __syncthreads();
+                continue;
+            }
             for (int n = sh.ob; n < sh.oe; n++) {
                 const int o = cload(A.offs, n);
                 float xv[RI][CPL];
@@ -853,7 +906,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             }
         }
 
-        if constexpr (LONG) long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, cmp, s_lsum, TRW * K);
+        if constexpr (LONG) long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, cmp, smem, TRW * K);
 
 #pragma unroll
         for (int i = 0; i < RI; i++) {
@@ -872,10 +925,10 @@ template <int K, typename XT, int RI, bool LONG>
 __global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
                                                         float *__restrict__ Y, int ldy, int ntiles)
 {
-    extern __shared__ float s_lsum[];
+    extern __shared__ float smem[];
     int tb, te;
     block_units(ntiles, &tb, &te);
-    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG>(A, X, ldx, Y, ldy, t, s_lsum);
+    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG>(A, X, ldx, Y, ldy, t, smem);
 }
 
 // LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
@@ -1012,11 +1065,11 @@ template <int K, typename XT>
 __global__ __launch_bounds__(kWG) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
                                                          int nd, int total)
 {
-    extern __shared__ float s_lsum[];
+    extern __shared__ float smem[];
     __shared__ int s_ub[kBatchLdsDesc];
     batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
-        if (w.A.L > 0) wide_tile<K, XT, kBatchRI, true>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, s_lsum);
-        else wide_tile<K, XT, kBatchRI, false>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, s_lsum);
+        if (w.A.L > 0) wide_tile<K, XT, kBatchRI, true>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem);
+        else wide_tile<K, XT, kBatchRI, false>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem);
     });
 }
 
@@ -1281,10 +1334,11 @@ int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
     const int ntiles = (A.M + TRW - 1) / TRW;
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
+    const size_t lds = (A.S > 0 || A.L > 0) ? (size_t)(TRW + kWideHalo) * K * 4 : 0;
     if (A.L > 0)
-        spmv_wide_kernel<K, XT, RI, true><<<G, kWG, (size_t)TRW * K * 4, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
+        spmv_wide_kernel<K, XT, RI, true><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
     else
-        spmv_wide_kernel<K, XT, RI, false><<<G, kWG, 0, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
+        spmv_wide_kernel<K, XT, RI, false><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
     HIPCHECK(hipGetLastError());
     return TC_OK;
 }
@@ -1392,7 +1446,7 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
         const size_t lds = ((size_t)kBatchCapv + 2 * TRW) * 4;
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     } else if (kern == 1) {
-        const size_t lds = (size_t)wide_rows(K, kBatchRI) * K * 4;
+        const size_t lds = (size_t)(wide_rows(K, kBatchRI) + kWideHalo) * K * 4;
         if (K == 2) batch_wide_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
         else if (K == 4) batch_wide_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
         else batch_wide_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
