@@ -52,6 +52,13 @@ namespace {
 constexpr int kWG = 256;
 constexpr int kWaves = kWG / 64;
 constexpr int kStageU = 8;  // loads in flight per lane when staging streams
+// Diagonal terms of a shape are taken in blocks of kDiagBlock offsets (by
+// offset index from the shape's first): a block is summed with plain FP32
+// adds in offset order and its sum enters the compensated accumulator once.
+// A block of 4 adds at most 3 ulps-relative error to its terms; the
+// compensated part keeps the cancellation against the remainder's zero
+// entries exact.
+constexpr int kDiagBlock = 4;
 
 struct alignas(16) ShapeDev {
     int xs, xe, ys, ob, oe, omin, omax, pad;
@@ -251,8 +258,9 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
         __syncthreads();
     }
     // Offsets outer, pairs inner: each offset is one scalar load and the P
-    // reads behind it are independent. Every pair still adds its terms in
-    // offset order.
+    // reads behind it are independent. Each pair sums the terms of a block
+    // of kDiagBlock offsets in offset order with plain adds and adds the
+    // block sum to (acc, cmp) compensated (the canonical order).
     bool act[P];
     int base[P];
     bool inner = true;  // all of this lane's pairs active, all columns inside the square
@@ -268,30 +276,54 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
 #pragma unroll
         for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
         if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
-            for (int n = sh.ob; n < sh.oe; n++) {
-                const int o = cload(A.offs, n) * K;
+            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                const int n1 = min(n0 + kDiagBlock, sh.oe);
+                float bs[P];
 #pragma unroll
-                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], s_val[li[p] + o]);
+                for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                for (int n = n0; n < n1; n++) {
+                    const int o = cload(A.offs, n) * K;
+#pragma unroll
+                    for (int p = 0; p < P; p++) bs[p] += s_val[li[p] + o];
+                }
+#pragma unroll
+                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
             }
         } else {
-            for (int n = sh.ob; n < sh.oe; n++) {
-                const int o = cload(A.offs, n);
+            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                const int n1 = min(n0 + kDiagBlock, sh.oe);
+                float bs[P];
 #pragma unroll
-                for (int p = 0; p < P; p++) {
-                    const int c = base[p] + o;
-                    if (act[p] && c >= clo && c < chi) cadd(acc[p], cmp[p], s_val[li[p] + o * K]);
+                for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                for (int n = n0; n < n1; n++) {
+                    const int o = cload(A.offs, n);
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        const int c = base[p] + o;
+                        if (act[p] && c >= clo && c < chi) bs[p] += s_val[li[p] + o * K];
+                    }
                 }
+#pragma unroll
+                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
             }
         }
         __syncthreads();
     } else {
-        for (int n = sh.ob; n < sh.oe; n++) {
-            const int o = cload(A.offs, n);
+        for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+            const int n1 = min(n0 + kDiagBlock, sh.oe);
+            float bs[P];
 #pragma unroll
-            for (int p = 0; p < P; p++) {
-                const int c = base[p] + o;
-                if (act[p] && c >= clo && c < chi) cadd(acc[p], cmp[p], load_x<XT>(X, (size_t)c * ldx + pcol[p]));
+            for (int p = 0; p < P; p++) bs[p] = 0.0f;
+            for (int n = n0; n < n1; n++) {
+                const int o = cload(A.offs, n);
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    const int c = base[p] + o;
+                    if (act[p] && c >= clo && c < chi) bs[p] += load_x<XT>(X, (size_t)c * ldx + pcol[p]);
+                }
             }
+#pragma unroll
+            for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
         }
     }
 }
@@ -843,46 +875,59 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     }
                 }
                 __syncthreads();
-                for (int n = sh.ob; n < sh.oe; n++) {
-                    const int o = cload(A.offs, n);
+                for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                    const int n1 = min(n0 + kDiagBlock, sh.oe);
+                    float bs[P];
 #pragma unroll
-                    for (int i = 0; i < RI; i++) {
-                        const int r = u0 + GPW * i + g;
-                        const int c = r + shift + o;
-                        if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
-                            const float *sp = smem + (c - w0) * K + c0;
-                            float xv[CPL];
-                            if constexpr (CPL == 4) {
-                                const float4 q = *(const float4 *)sp;
-                                xv[0] = q.x, xv[1] = q.y, xv[2] = q.z, xv[3] = q.w;
-                            } else {
-                                const float2 q = *(const float2 *)sp;
-                                xv[0] = q.x, xv[1] = q.y;
+                    for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                    for (int n = n0; n < n1; n++) {
+                        const int o = cload(A.offs, n);
+#pragma unroll
+                        for (int i = 0; i < RI; i++) {
+                            const int r = u0 + GPW * i + g;
+                            const int c = r + shift + o;
+                            if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
+                                const float *sp = smem + (c - w0) * K + c0;
+                                if constexpr (CPL == 4) {
+                                    const float4 q = *(const float4 *)sp;
+                                    bs[i * 4] += q.x, bs[i * 4 + 1] += q.y, bs[i * 4 + 2] += q.z, bs[i * 4 + 3] += q.w;
+                                } else {
+                                    const float2 q = *(const float2 *)sp;
+                                    bs[i * 2] += q.x, bs[i * 2 + 1] += q.y;
+                                }
                             }
-#pragma unroll
-                            for (int m = 0; m < CPL; m++) cadd(acc[i * CPL + m], cmp[i * CPL + m], xv[m]);
                         }
                     }
+#pragma unroll
+                    for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
                 }
                 __syncthreads();
                 continue;
             }
-            for (int n = sh.ob; n < sh.oe; n++) {
-                const int o = cload(A.offs, n);
-                float xv[RI][CPL];
-                bool in[RI];
+            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                const int n1 = min(n0 + kDiagBlock, sh.oe);
+                float bs[P];
 #pragma unroll
-                for (int i = 0; i < RI; i++) {
-                    const int r = u0 + GPW * i + g;
-                    const int c = r - sh.xs + sh.ys + o;
-                    in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
-                    if (in[i]) load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                for (int n = n0; n < n1; n++) {
+                    const int o = cload(A.offs, n);
+                    float xv[RI][CPL];
+                    bool in[RI];
+#pragma unroll
+                    for (int i = 0; i < RI; i++) {
+                        const int r = u0 + GPW * i + g;
+                        const int c = r - sh.xs + sh.ys + o;
+                        in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
+                        if (in[i]) load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                    }
+#pragma unroll
+                    for (int i = 0; i < RI; i++)
+                        if (in[i])
+#pragma unroll
+                            for (int m = 0; m < CPL; m++) bs[i * CPL + m] += xv[i][m];
                 }
 #pragma unroll
-                for (int i = 0; i < RI; i++)
-                    if (in[i])
-#pragma unroll
-                        for (int m = 0; m < CPL; m++) cadd(acc[i * CPL + m], cmp[i * CPL + m], xv[i][m]);
+                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
             }
         }
 
@@ -1059,7 +1104,7 @@ __global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restr
 }
 
 // K > 1 chunks whose X and Y rows split into aligned vectors: wide-rows tiles
-constexpr int kBatchRI = 2;
+constexpr int kBatchRI = 1;  // one row per lane group: 42-56 VGPRs, full occupancy
 
 template <int K, typename XT>
 __global__ __launch_bounds__(kWG) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
@@ -1134,11 +1179,12 @@ int pick_k(int k)
     return 1;
 }
 
-// Tiles of 512 rows (k = 1) or 512 (row, column) pairs (k > 1) and a
-// 4096-value stage: the best or near-best setting measured on R-MAT and
-// banded matrices. Matrices whose remainder averages more than 8 entries a
-// row (banded with many zero cells) use 256-row tiles at k = 1, so a tile's
-// entries pass through the stage in about two rounds.
+// k = 1: tiles of 512 rows and a 4096-value stage; k > 1: wide-rows tiles
+// of one row per lane group (full occupancy): the best or near-best settings
+// measured on R-MAT, banded and stencil matrices. Matrices whose remainder
+// averages more than 8 entries a row (banded with many zero cells) use
+// 256-row tiles at k = 1, so a tile's entries pass through the stage in
+// about two rounds.
 // k > 1: the wide-rows kernel serves K in {2, 4, 8, 16, 32, 64} unless
 // config flag bit 2 (kStagedK) asks for the staged tile kernel, which also
 // serves every other K.
@@ -1155,7 +1201,7 @@ tc_config default_config(int K, long long remainder, int M)
 {
     tc_config c{};
     const bool heavy = M > 0 && remainder > 8ll * M;
-    c.rows_per_tile = K == 1 ? (heavy ? 256 : 512) : wide_k(K) ? wide_rows(K, 2) : (kWG * 2) / K;
+    c.rows_per_tile = K == 1 ? (heavy ? 256 : 512) : wide_k(K) ? wide_rows(K, 1) : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
     c.flags = K == 1 ? kQuadOff : 0;  // quad rows measured no faster on R-MAT; the autotuner tries both

@@ -352,3 +352,51 @@ def test_taichi_spmv_host_path(dev):
     y, t_ms, s_ms = T.taichi_spmv(fmt, x, warmup=2, runs=5)
     assert np.array_equal(y, O.csr_spmv_f32(rp, col, x))
     assert s_ms > 0
+
+
+@pytest.mark.parametrize("case", ["banded", "long"])
+def test_masked_column_chunks(case, dev):
+    """k = 3, 5, 6, 7 (and 1..8) columns of an 8-column buffer run as one
+    masked wide-rows launch: columns past k are neither read nor written,
+    each column equals the single-vector result, and the batched launch over
+    the same views equals the plain one."""
+    if case == "banded":
+        M = N = 20011
+        rp, col = T.gen_banded(M, 13, 0.7, 61)
+    else:
+        M = N = 50003
+        rp, col = _local_mix(M, 1, 40, 509, 200, 20000, 62)
+    fmt = host_format(M, N, rp, col)
+    A = T.Matrix(fmt)
+    X8 = torch.from_numpy(T.gen_vector(N * 8, 1, 70).reshape(N, 8)).to(dev)
+    cols1 = []
+    for c in range(8):
+        y = torch.empty(M, device=dev)
+        A.spmv(X8[:, c].contiguous(), y)
+        cols1.append(y)
+    torch.cuda.synchronize()
+    items = []
+    for k in range(1, 9):
+        Y8 = torch.full((M, 8), float("nan"), device=dev)
+        A.spmv(X8[:, :k], Y8[:, :k])
+        torch.cuda.synchronize()
+        assert torch.isnan(Y8[:, k:]).all(), k                # nothing written past k
+        for c in range(k):
+            assert torch.equal(Y8[:, c], cols1[c]), (k, c)
+        assert np.array_equal(Y8[:, 0].cpu().numpy(), O.csr_spmv_f32(rp, col, X8[:, 0].cpu().numpy()))
+        Yb = torch.full((M, 8), float("nan"), device=dev)
+        items.append((A, X8[:, :k], Yb[:, :k], Y8, Yb, k))
+    T.spmv_batch([(a, x, y) for a, x, y, _, _, _ in items])
+    torch.cuda.synchronize()
+    for _, _, _, Y8, Yb, k in items:
+        assert torch.equal(Yb[:, :k], Y8[:, :k]), k
+        assert torch.isnan(Yb[:, k:]).all(), k
+    # the last rows of a tight buffer: X of exactly k columns at an offset
+    # that keeps rows aligned (ldx = 8, no column past k exists in memory)
+    for k in (3, 5, 7):
+        Xt = X8[:, :k].contiguous()                           # ldx = k: unaligned -> staged
+        Yt = torch.full((M, k), float("nan"), device=dev)
+        A.spmv(Xt, Yt)
+        torch.cuda.synchronize()
+        for c in range(k):
+            assert torch.equal(Yt[:, c], cols1[c]), (k, c)
