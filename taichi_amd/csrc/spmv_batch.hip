@@ -1,0 +1,271 @@
+// Batched SpMV over many requests on any mix of resident matrices: the
+// requests are cut into column chunks, grouped by (kernel, chunk width, x
+// type), and each group is one launch over all its tiles.
+#include "batch.h"
+#include "spmv_host.h"
+
+namespace tcd {
+
+// ---- batched launches over many requests ----
+// A batch is cut into column chunks of K in {8, 4, 2, 1} columns and the
+// chunks are grouped by (K, x type); each group is one launch of a kernel
+// specialised for it, so no launch carries the registers of the others. The
+// tiles of a group are numbered consecutively, chunk by chunk; a workgroup
+// finds its chunk by binary search over the group's tile starts (staged in
+// LDS) and reads the chunk's descriptor with scalar loads. Column chunks
+// change nothing in the result: the per-(row, column) order is independent
+// of K.
+constexpr int kBatchP = 2;       // K > 1: run_tile with P = 2 (512/K rows a tile)
+constexpr int kBatchRPT = 2;     // K = 1: rows_tile with 512 rows a tile
+constexpr int kBatchCapv = 4096; // staged values per pass
+constexpr int kBatchLdsDesc = 512;
+
+struct WorkDesc {
+    MatView A;
+    const void *X;
+    float *Y;
+    int ldx, ldy;
+    int unit_begin;
+    int fast;  // one shape with one offset: the rows tile's FAST path
+};
+
+__device__ __forceinline__ WorkDesc load_desc(const WorkDesc *D, int d)
+{
+    constexpr int NW = sizeof(WorkDesc) / 4;
+    union {
+        WorkDesc w;
+        int u[NW];
+    } t;
+    const int *p = (const int *)(D + d);
+#pragma unroll
+    for (int i = 0; i < NW; i++) t.u[i] = cload(p, i);
+    return t.w;
+}
+
+// Runs f(desc, local_tile) for every tile of this workgroup's range of the
+// group's `total` tiles. UB[i] is chunk i's first tile (ascending).
+template <typename F>
+__device__ __forceinline__ void batch_loop(const WorkDesc *__restrict__ D, const int *__restrict__ UB, int nd,
+                                           int total, int *s_ub, F &&f)
+{
+    int ub, ue;
+    block_units(total, &ub, &ue);
+    if (ub >= ue) return;
+    const bool in_lds = nd <= kBatchLdsDesc;
+    if (in_lds) {
+        for (int i = threadIdx.x; i < nd; i += kWG) s_ub[i] = UB[i];
+        __syncthreads();
+    }
+    int lo = 0, hi = nd - 1;  // last chunk with UB <= ub
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((in_lds ? s_ub[mid] : cload(UB, mid)) <= ub) lo = mid;
+        else hi = mid - 1;
+    }
+    int d = __builtin_amdgcn_readfirstlane(lo);
+    for (int u = ub; u < ue; u++) {
+        while (d + 1 < nd && (in_lds ? s_ub[d + 1] : cload(UB, d + 1)) <= u) d++;
+        d = __builtin_amdgcn_readfirstlane(d);
+        const WorkDesc w = load_desc(D, d);
+        f(w, u - w.unit_begin);
+        __syncthreads();
+    }
+}
+
+template <typename XT>
+__global__ __launch_bounds__(kWG) void batch_rows_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
+                                                         int nd, int total, int capv)
+{
+    extern __shared__ float smem[];
+    __shared__ int s_ub[kBatchLdsDesc];
+    batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
+        const XT *X = (const XT *)w.X;
+        const bool lng = w.A.L > 0;
+        if (w.A.ow == 1) {
+            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 1, true, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, kBatchRPT, 1, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, kBatchRPT, 1, false, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, kBatchRPT, 1, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+        } else {
+            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 2, true, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, kBatchRPT, 2, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, kBatchRPT, 2, false, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, kBatchRPT, 2, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+        }
+    });
+}
+
+template <int K, typename XT>
+__global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
+                                                         int nd, int total, int capv)
+{
+    constexpr int TRW = (kWG * kBatchP) / K;
+    extern __shared__ float smem[];
+    __shared__ int s_ub[kBatchLdsDesc];
+    float *s_val = smem;
+    float *s_lsum = s_val + capv;
+    int *s_start = (int *)(s_lsum + kWG * kBatchP);
+    batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
+        run_tile<K, XT, kBatchP>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t * TRW, s_val, capv, s_start, s_lsum);
+    });
+}
+
+// K > 1 chunks whose X and Y rows split into aligned vectors: wide-rows tiles
+constexpr int kBatchRI = 1;  // one row per lane group: 42-56 VGPRs, full occupancy
+
+template <int K, typename XT>
+__global__ __launch_bounds__(kWG) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
+                                                         int nd, int total)
+{
+    extern __shared__ float smem[];
+    __shared__ int s_ub[kBatchLdsDesc];
+    batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
+        if (w.A.L > 0) wide_tile<K, XT, kBatchRI, true>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem);
+        else wide_tile<K, XT, kBatchRI, false>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem);
+    });
+}
+
+}  // namespace tcd
+
+namespace tc {
+
+using namespace tcd;
+
+// chunks of one request of k columns: 8s, then 4, 2, 1 as needed
+int chunk_count(int k) { return k / 8 + ((k & 4) != 0) + ((k & 2) != 0) + (k & 1); }
+
+constexpr int kMaxBatchK = 64;
+
+size_t batch_desc_bytes(int max_requests)
+{
+    const size_t chunks = (size_t)max_requests * chunk_count(kMaxBatchK - 1);
+    return chunks * sizeof(WorkDesc) + chunks * sizeof(int) + 16;
+}
+
+// Batch classes: (kernel, chunk width, x type). Kernel 0: rows tiles (K = 1);
+// 1: wide-rows tiles (K > 1, aligned); 2: staged tiles (K > 1, otherwise).
+constexpr int kBatchClasses = 3 * 4 * 3;
+inline int batch_class(int kern, int ki, int dt) { return (kern * 4 + ki) * 3 + dt; }
+
+int batch_tile_rows(int kern, int K)
+{
+    if (kern == 0) return kWG * kBatchRPT;
+    if (kern == 1) return wide_rows(K, kBatchRI);
+    return (kWG * kBatchP) / K;
+}
+
+template <typename XT>
+void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int nd, int units, hipStream_t s)
+{
+    const int grid = units;  // one workgroup per tile: the dispatcher balances chunks of any size
+    if (kern == 0) {
+        constexpr int TRW = kWG * kBatchRPT;
+        const size_t lds = ((size_t)kBatchCapv + 2 * TRW) * 4;
+        batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+    } else if (kern == 1) {
+        const size_t lds = (size_t)(wide_rows(K, kBatchRI) + kWideHalo) * K * 4;
+        if (K == 2) batch_wide_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+        else if (K == 4) batch_wide_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+        else batch_wide_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+    } else {
+        const size_t lds = lds_bytes(kBatchCapv, kBatchP, (kWG * kBatchP) / K);
+        if (K == 2) batch_tile_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+        else if (K == 4) batch_tile_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+        else batch_tile_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+    }
+}
+
+// chunk c0..c0+K of request r: its batch class
+int chunk_class(const tc_request &r, int ki, int K, int c0, int dt)
+{
+    if (K == 1) return batch_class(0, ki, dt);
+    const int es = elem_size(r.dtype), cpl = std::min(K, 4);
+    const uintptr_t xa = (uintptr_t)r.X + (uintptr_t)c0 * es, ya = (uintptr_t)(r.Y + c0);
+    const bool aligned = r.ldx % cpl == 0 && r.ldy % cpl == 0 && xa % ((uintptr_t)cpl * es) == 0 &&
+                         ya % ((uintptr_t)cpl * 4) == 0;
+    return batch_class(aligned ? 1 : 2, ki, dt);
+}
+
+// Fills the chunk descriptors of n requests into host_desc (at least
+// batch_desc_bytes(n) bytes, valid until the copy on s has run), copies them
+// to dev_desc and enqueues one launch per batch class present. Each
+// request's y equals what tc_spmv computes for it alone.
+int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s)
+{
+    static const int kChunkK[4] = {1, 2, 4, 8};
+    // chunks of a request: 8s, then 4, 2, 1 as k's bits say; calls
+    // f(ki, K, c0) for each, widest first
+    auto chunks = [](int k, auto &&f) {
+        int c0 = 0;
+        while (k - c0 >= 8) {
+            f(3, 8, c0);
+            c0 += 8;
+        }
+        for (int ki = 2; ki >= 0; ki--)
+            if ((k - c0) & (1 << ki)) {
+                f(ki, 1 << ki, c0);
+                c0 += 1 << ki;
+            }
+    };
+    int cls_count[kBatchClasses] = {0};
+    int nchunks = 0;
+    for (int i = 0; i < n; i++) {
+        const tc_request &r = reqs[i];
+        int rc = check_spmv_args(r.A, r.X, r.ldx, r.dtype, r.Y, r.ldy, r.k);
+        if (rc != TC_OK) return rc;
+        if (r.k > kMaxBatchK) return tc::fail(TC_ERR_UNSUPPORTED, "batch: k=%d > %d", r.k, kMaxBatchK);
+        const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
+        chunks(r.k, [&](int ki, int K, int c0) {
+            cls_count[chunk_class(r, ki, K, c0, dt)]++;
+            nchunks++;
+        });
+    }
+    if (nchunks == 0) return TC_OK;
+    int cls_begin[kBatchClasses + 1] = {0};
+    for (int c = 0; c < kBatchClasses; c++) cls_begin[c + 1] = cls_begin[c] + cls_count[c];
+    WorkDesc *D = (WorkDesc *)host_desc;
+    const size_t ub_off = ((size_t)nchunks * sizeof(WorkDesc) + 15) & ~(size_t)15;
+    int *UB = (int *)((char *)host_desc + ub_off);
+    int fill[kBatchClasses];
+    long long units[kBatchClasses] = {0};
+    for (int c = 0; c < kBatchClasses; c++) fill[c] = cls_begin[c];
+    // chunks in request order within each class
+    for (int i = 0; i < n; i++) {
+        const tc_request &r = reqs[i];
+        const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
+        const MatView &v = r.A->v;
+        const bool fast = v.S == 1 && v.sh0.oe - v.sh0.ob == 1;
+        chunks(r.k, [&](int ki, int K, int c0) {
+            const int cl = chunk_class(r, ki, K, c0, dt);
+            WorkDesc &w = D[fill[cl]];
+            w.A = v;
+            w.X = (const char *)r.X + (size_t)c0 * elem_size(r.dtype);
+            w.Y = r.Y + c0;
+            w.ldx = r.ldx;
+            w.ldy = r.ldy;
+            w.fast = fast ? 1 : 0;
+            const int trw = batch_tile_rows(cl / 12, K);
+            w.unit_begin = (int)units[cl];
+            UB[fill[cl]] = w.unit_begin;
+            units[cl] += (v.M + trw - 1) / trw;
+            fill[cl]++;
+        });
+    }
+    for (int c = 0; c < kBatchClasses; c++)
+        if (units[c] > INT32_MAX) return tc::fail(TC_ERR_UNSUPPORTED, "batch: too many tiles");
+    HIPCHECK(hipMemcpyAsync(dev_desc, host_desc, ub_off + (size_t)nchunks * sizeof(int), hipMemcpyHostToDevice, s));
+    const WorkDesc *dD = (const WorkDesc *)dev_desc;
+    const int *dUB = (const int *)((const char *)dev_desc + ub_off);
+    for (int c = 0; c < kBatchClasses; c++) {
+        if (units[c] == 0) continue;
+        const int kern = c / 12, K = kChunkK[(c / 3) % 4], dt = c % 3, nd = cls_count[c], b = cls_begin[c];
+        if (dt == 0) launch_batch_class<float>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
+        else if (dt == 1) launch_batch_class<bf16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
+        else launch_batch_class<f16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
+        HIPCHECK(hipGetLastError());
+    }
+    return TC_OK;
+}
+
+}  // namespace tc
+

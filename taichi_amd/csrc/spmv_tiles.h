@@ -1,0 +1,783 @@
+// Device side of the hybrid diagonal + CSR SpMV for gfx950: the device
+// layout of a resident matrix and the per-tile routines every SpMV kernel
+// (spmv_k1.hip, spmv_wide.hip, spmv_staged.hip, spmv_batch.hip) is built from.
+//
+// Replaces the device half of taichi_SpMV and CSR_coop_spmv_kernel{,_hybrid1,2,3}
+// (TaiChi.cu:965-1456). y = A x with A in the TaiChi_new format: dense
+// diagonal segments of square shapes plus a value-free, sign-encoded CSR
+// remainder (csrCol = col<<1 | sign; sign 0 subtracts x, compensating a zero
+// inside a dense diagonal).
+//
+// Device layout (built by tc_matrix_create from a host TaiChi_new):
+//   shapes[S]   {xStart, xStop, yStart, off_begin, off_end, off_min, off_max}
+//   offs[D]     dense diagonal offsets
+//   gbase[G+1]  start of each 64-row group's short-row entries in cols
+//   roff[M]     u8 or u16: end of row i's short entries, relative to gbase[i/64]
+//   cols[R]     encoded entries: short rows in row order, then long rows
+//   glong[G+1], lrow[L], lptr[L+1]   rows longer than the long-row threshold
+// Compared with the reference's int rowptr (4 bytes per row) the per-row
+// cost is 1 (or 2) bytes plus 4 bytes per 64 rows.
+//
+// A workgroup (4 wavefronts) computes a tile of rows. Every kernel adds a
+// (row, column) pair's terms in one canonical order fixed by the format:
+//   1. diagonal terms, shape by shape; within a shape the offsets in blocks of
+//      kDiagBlock by offset index, each block summed with plain adds and the
+//      block sum added compensated; the tile's x window is staged in LDS when
+//      a shape has two or more offsets, so no column index is read for it;
+//   2. remainder entries in stored order (streamed with coalesced loads, x
+//      gathered and signed, staged in LDS, summed by one lane per row);
+//   3. long rows: one wavefront each, 64-lane strided partial sums and a
+//      fixed DPP/ds_swizzle butterfly.
+// The order never depends on the tile size, the grid, the column chunking or
+// the batch a request runs in, so a request gives bit-identical y alone, in a
+// batch, or with another configuration.
+//
+// No MFMA: one add per stored index, bound by HBM bandwidth.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tcd {
+
+constexpr int kWG = 256;
+constexpr int kWaves = kWG / 64;
+constexpr int kStageU = 8;  // loads in flight per lane when staging streams
+// Diagonal terms of a shape are taken in blocks of kDiagBlock offsets (by
+// offset index from the shape's first): a block is summed with plain FP32
+// adds in offset order and its sum enters the compensated accumulator once.
+// A block of 4 adds at most 3 ulps-relative error to its terms; the
+// compensated part keeps the cancellation against the remainder's zero
+// entries exact.
+constexpr int kDiagBlock = 4;
+
+struct alignas(16) ShapeDev {
+    int xs, xe, ys, ob, oe, omin, omax, pad;
+};
+
+// Device pointers and sizes of one resident matrix; passed by value.
+struct MatView {
+    int M, N, S, L;
+    int ow;        // bytes per local row offset: 1 or 2
+    int ngroups;
+    ShapeDev sh0;  // copy of shapes[0]: arrives with the kernel arguments
+    const ShapeDev *shapes;
+    const int *offs;
+    const int *gbase;
+    const void *roff;
+    const int *cols;
+    const int *glong;
+    const int *lrow;
+    const int *lptr;
+};
+
+struct bf16_t {
+    uint16_t bits;
+};
+typedef _Float16 f16_t;
+
+template <typename XT>
+__device__ __forceinline__ float load_x(const XT *__restrict__ X, size_t i);
+template <>
+__device__ __forceinline__ float load_x<float>(const float *__restrict__ X, size_t i)
+{
+    return X[i];
+}
+template <>
+__device__ __forceinline__ float load_x<bf16_t>(const bf16_t *__restrict__ X, size_t i)
+{
+    return __uint_as_float((uint32_t)X[i].bits << 16);
+}
+template <>
+__device__ __forceinline__ float load_x<f16_t>(const f16_t *__restrict__ X, size_t i)
+{
+    return (float)X[i];
+}
+
+// Read-only format metadata (group bases, shapes, offsets, long-row tables)
+// is read through the constant address space: uniform indices then become
+// scalar loads, which do not make the vector loads behind them wait.
+template <typename T>
+__device__ __forceinline__ T cload(const T *p, int i)
+{
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
+__device__ __forceinline__ ShapeDev load_shape(const MatView &A, int s)
+{
+    const int *p = (const int *)(A.shapes + s);
+    ShapeDev h;
+    h.xs = cload(p, 0);
+    h.xe = cload(p, 1);
+    h.ys = cload(p, 2);
+    h.ob = cload(p, 3);
+    h.oe = cload(p, 4);
+    h.omin = cload(p, 5);
+    h.omax = cload(p, 6);
+    h.pad = 0;
+    return h;
+}
+
+// start of row i's short entries (i in [0, M]); row i ends at row_start(i+1)
+__device__ __forceinline__ int row_start(const MatView &A, int i)
+{
+    int b = cload(A.gbase, i >> 6);
+    if (i & 63) b += A.ow == 1 ? (int)((const uint8_t *)A.roff)[i - 1] : (int)((const uint16_t *)A.roff)[i - 1];
+    return b;
+}
+
+// Sum over the 64 lanes of a wavefront in a fixed order: quad swaps and
+// row mirrors by DPP, the 16-lane swap by ds_swizzle, halves by readlane.
+__device__ __forceinline__ float wave_sum(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
+    v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));                      // lane ^ 16
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+}
+
+// Compensated accumulation: s + c carries the running sum with the exact
+// rounding error of every add (Knuth's TwoSum, branch-free) kept in c, the
+// same error term Neumaier's magnitude-ordered form computes. Dense diagonals
+// add x over whole segments and the remainder subtracts it again at their
+// zero cells, so plain FP32 sums lose accuracy to cancellation; the
+// compensated sum keeps each y within a few ulps of the exact sum.
+// Integer-valued inputs stay exact (c stays 0). Needs -ffp-contract=off.
+__device__ __forceinline__ void cadd(float &s, float &c, float x)
+{
+    const float t = s + x;
+    const float z = t - s;
+    c += (s - (t - z)) + (x - z);
+    s = t;
+}
+
+// Bijection of block ids so that each XCD (blocks are dispatched round-robin
+// over the 8 XCDs) gets a contiguous range of slots, hence contiguous rows
+// and x windows in its own L2.
+__device__ __forceinline__ int xcd_slot(int b, int G)
+{
+    const int q = G >> 3, r = G & 7, x = b & 7, l = b >> 3;
+    return x * q + min(x, r) + l;
+}
+
+// Contiguous range [*b, *e) of `total` work units for this block. With one
+// block per unit it is just the XCD-remapped block id; otherwise a 32-bit
+// split (callers keep total * G below 2^31).
+__device__ __forceinline__ void block_units(int total, int *b, int *e)
+{
+    const int G = gridDim.x;
+    const int v = xcd_slot(blockIdx.x, G);
+    if (G == total) {
+        *b = v;
+        *e = v + 1;
+    } else {
+        *b = (int)((unsigned)v * (unsigned)total / (unsigned)G);
+        *e = (int)((unsigned)(v + 1) * (unsigned)total / (unsigned)G);
+    }
+}
+
+// Stage the signed x values of ne contiguous entries (K columns each) into
+// s_val[j*K + c]. The loads of each batch (U items per lane) are all issued
+// before the first gather waits on them; lanes past the end re-read a valid
+// item and do not store.
+template <int K, typename XT, int U = kStageU>
+__device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne, const XT *__restrict__ X, int ldx,
+                                              float *s_val)
+{
+    const int tid = threadIdx.x;
+    const int n = ne * K;
+    for (int b = 0; b < n; b += kWG * U) {
+        // slots past the end of the stream are skipped as a whole (the
+        // condition is uniform across the workgroup)
+        int v[U], cc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            v[u] = 1;
+            cc[u] = 0;
+            if (b + u * kWG < n) {
+                const int idx = min(b + u * kWG + tid, n - 1);
+                const int j = idx / K;
+                cc[u] = idx - j * K;
+                v[u] = cp[j];
+            }
+        }
+        float xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) xv[u] = b + u * kWG < n ? load_x<XT>(X, (size_t)(v[u] >> 1) * ldx + cc[u]) : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int idx = b + u * kWG + tid;
+            if (idx < n) s_val[idx] = (v[u] & 1) ? xv[u] : -xv[u];
+        }
+    }
+}
+
+// Diagonal terms of one shape for the tile's pairs, in canonical order
+// (offsets as stored), added to (acc, cmp). The x window of a shape with two
+// or more offsets is staged in LDS once per tile (s_val, left free after).
+template <int K, typename XT, int P>
+__device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
+                                               const ShapeDev &sh, int ra, int rb, const int (&prow)[P],
+                                               const int (&pcol)[P], const bool (&pv)[P], float (&acc)[P],
+                                               float (&cmp)[P], float *s_val, int capv)
+{
+    const int tid = threadIdx.x;
+    const int shift = sh.ys - sh.xs;
+    const int clo = sh.ys, chi = sh.ys + (sh.xe - sh.xs);
+    const int w0 = max(clo, ra + shift + sh.omin);
+    const int w1 = min(chi, rb - 1 + shift + sh.omax + 1);
+    const bool stage = (sh.oe - sh.ob) >= 2 && w1 > w0 && (w1 - w0) * K <= capv;
+    if (stage) {
+        const int nw = (w1 - w0) * K;
+        for (int b = 0; b < nw; b += kWG * kStageU) {
+            float t[kStageU];
+#pragma unroll
+            for (int u = 0; u < kStageU; u++) {
+                const int idx = min(b + u * kWG + tid, nw - 1);
+                const int cc = idx / K, c = idx - cc * K;
+                t[u] = load_x<XT>(X, (size_t)(w0 + cc) * ldx + c);
+            }
+#pragma unroll
+            for (int u = 0; u < kStageU; u++) {
+                const int idx = b + u * kWG + tid;
+                if (idx < nw) s_val[idx] = t[u];
+            }
+        }
+        __syncthreads();
+    }
+    // Offsets outer, pairs inner: each offset is one scalar load and the P
+    // reads behind it are independent. Each pair sums the terms of a block
+    // of kDiagBlock offsets in offset order with plain adds and adds the
+    // block sum to (acc, cmp) compensated (the canonical order).
+    bool act[P];
+    int base[P];
+    bool inner = true;  // all of this lane's pairs active, all columns inside the square
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int r = u0 + prow[p];
+        act[p] = pv[p] && r >= ra && r < rb;
+        base[p] = r + shift;
+        inner = inner && act[p] && base[p] + sh.omin >= clo && base[p] + sh.omax < chi;
+    }
+    if (stage) {
+        int li[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
+        if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                const int n1 = min(n0 + kDiagBlock, sh.oe);
+                float bs[P];
+#pragma unroll
+                for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                for (int n = n0; n < n1; n++) {
+                    const int o = cload(A.offs, n) * K;
+#pragma unroll
+                    for (int p = 0; p < P; p++) bs[p] += s_val[li[p] + o];
+                }
+#pragma unroll
+                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+            }
+        } else {
+            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                const int n1 = min(n0 + kDiagBlock, sh.oe);
+                float bs[P];
+#pragma unroll
+                for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                for (int n = n0; n < n1; n++) {
+                    const int o = cload(A.offs, n);
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        const int c = base[p] + o;
+                        if (act[p] && c >= clo && c < chi) bs[p] += s_val[li[p] + o * K];
+                    }
+                }
+#pragma unroll
+                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+            }
+        }
+        __syncthreads();
+    } else {
+        for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+            const int n1 = min(n0 + kDiagBlock, sh.oe);
+            float bs[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) bs[p] = 0.0f;
+            for (int n = n0; n < n1; n++) {
+                const int o = cload(A.offs, n);
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    const int c = base[p] + o;
+                    if (act[p] && c >= clo && c < chi) bs[p] += load_x<XT>(X, (size_t)c * ldx + pcol[p]);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+        }
+    }
+}
+
+template <int K, typename XT, int P>
+__device__ __forceinline__ void diag_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
+                                           const int (&prow)[P], const int (&pcol)[P], const bool (&pv)[P],
+                                           float (&acc)[P], float (&cmp)[P], float *s_val, int capv)
+{
+    for (int s = 0; s < A.S; s++) {
+        const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
+        const int ra = max(u0, sh.xs), rb = min(u0 + rows, sh.xe);
+        if (ra >= rb) continue;
+        diag_one_shape<K, XT, P>(A, X, ldx, u0, s, sh, ra, rb, prow, pcol, pv, acc, cmp, s_val, capv);
+    }
+}
+
+// Long rows of the tile: one wavefront each, partial sums strided over the
+// 64 lanes and reduced by wave_sum; results land in s_lsum[local_row*K + c]
+// and are added to acc. Uniform control flow; leaves s_lsum free.
+template <int K, typename XT, int P>
+__device__ __forceinline__ void long_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
+                                           const int (&lq)[P], const bool (&pv)[P], float (&acc)[P],
+                                           float (&cmp)[P], float *s_lsum, int nlsum)
+{
+    if (A.L == 0) return;
+    const int lb = cload(A.glong, u0 >> 6), le = cload(A.glong, ((u0 + rows - 1) >> 6) + 1);
+    if (lb >= le) return;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < nlsum; q += kWG) s_lsum[q] = 0.0f;
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int l = lb + wave; l < le; l += kWaves) {
+        const int row = cload(A.lrow, l);
+        if (row < u0 || row >= u0 + rows) continue;
+        const int j0 = cload(A.lptr, l), j1 = cload(A.lptr, l + 1);
+        for (int c = 0; c < K; c++) {
+            float part = 0.0f, pe = 0.0f;
+            for (int j = j0 + lane; j < j1; j += 64) {
+                const int v = A.cols[j];
+                const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
+                cadd(part, pe, (v & 1) ? xv : -xv);
+            }
+            part = wave_sum(part + pe);
+            if (lane == 0) s_lsum[(row - u0) * K + c] = part;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < P; p++)
+        if (pv[p]) cadd(acc[p], cmp[p], s_lsum[lq[p]]);
+    __syncthreads();
+}
+
+// Canonical per-(row, column) order, shared by every kernel below, with
+// compensated accumulation (cadd) into (acc, cmp) and y = acc + cmp:
+//   diagonal terms (shape order, offset order);
+//   short-row entries in stored order, one lane, serially;
+//   the long-row wavefront sum (long rows have no short entries).
+template <int K, typename XT, int P>
+__device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
+                                         int ldy, int u0, float *s_val, int capv, int *s_start, float *s_lsum)
+{
+    constexpr int TRW = (kWG * P) / K;
+    constexpr int NP = TRW * K;
+    const int tid = threadIdx.x;
+    const int rows = min(TRW, A.M - u0);
+
+    // 1. row extents
+    for (int r = tid; r <= TRW; r += kWG) s_start[r] = row_start(A, min(u0 + r, A.M));
+
+    float acc[P], cmp[P];
+    int prow[P], pcol[P], lq[P];
+    bool pv[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const int q = tid + p * kWG;
+        prow[p] = q / K;
+        pcol[p] = q - prow[p] * K;
+        pv[p] = q < NP && prow[p] < rows;
+        lq[p] = q < NP ? q : 0;
+        acc[p] = 0.0f;
+        cmp[p] = 0.0f;
+    }
+
+    // 2. diagonal segments
+    diag_phase<K, XT, P>(A, X, ldx, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+    __syncthreads();
+    const int ebase = s_start[0];
+    const int E = s_start[TRW] - ebase;
+
+    // 3. short rows: stream entries, stage signed x values, serial row sums
+    const int cape = capv / K;
+    for (int e0 = 0; e0 < E; e0 += cape) {
+        const int ne = min(cape, E - e0);
+        stage_entries<K, XT>(A.cols + ebase + e0, ne, X, ldx, s_val);
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            if (!pv[p]) continue;
+            const int lo = max(s_start[prow[p]] - ebase, e0);
+            const int hi = min(s_start[prow[p] + 1] - ebase, e0 + ne);
+            float a = acc[p], e = cmp[p];
+            for (int j = lo; j < hi; j++) cadd(a, e, s_val[(j - e0) * K + pcol[p]]);
+            acc[p] = a;
+            cmp[p] = e;
+        }
+        __syncthreads();
+    }
+
+    // 4. long rows: one wavefront per row
+    long_phase<K, XT, P>(A, X, ldx, u0, rows, lq, pv, acc, cmp, s_lsum, NP);
+
+    // 5. store
+#pragma unroll
+    for (int p = 0; p < P; p++)
+        if (pv[p]) Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]] = acc[p] + cmp[p];
+}
+
+// ---- single right-hand side: lane-strided row kernel ----
+// A workgroup handles a tile of 256*RPT rows; lane l of wavefront w owns the
+// rows u0 + 256*j + 64*w + l (j < RPT), so every row-indexed access -- local
+// offsets, diagonal x, y -- is one coalesced dword per lane, a wavefront's 64
+// rows form exactly one offset group (their base is one scalar load), and a
+// row's first entry is its left neighbour's end. The tile's entry range is
+// two scalar loads (tiles are 64-row aligned). All loads of a phase are
+// issued before the first use. Same canonical order as run_tile. Column c of
+// X is X[c * ldx], row r of Y is Y[r * ldy] (1 and 1 for a plain vector).
+// LDS: s_val[capv] | s_lsum[TRW] (LONG) | s_win[TRW] (FAST).
+template <typename XT, int RPT, int OW, bool FAST, bool LONG>
+__device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
+                                          int ldy, int t, float *smem, int capv)
+{
+    constexpr int TRW = kWG * RPT;
+    float *s_val = smem;
+    float *s_lsum = smem + capv;
+    float *s_win = smem + capv + (LONG ? TRW : 0);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint8_t *ro8 = (const uint8_t *)A.roff;
+    const uint16_t *ro16 = (const uint16_t *)A.roff;
+    {
+        const int u0 = t * TRW;
+        const int rows = min(TRW, A.M - u0);
+        const int ebase = cload(A.gbase, u0 >> 6);
+        const int E = cload(A.gbase, u0 + TRW >= A.M ? A.ngroups : (u0 + TRW) >> 6) - ebase;
+
+        // Issue order: x window (FAST), row offsets, the first batch of the
+        // column stream; the adds follow in the canonical order.
+        constexpr int U = 4;
+        int prow[RPT], pcol[RPT], lq[RPT], endv[RPT], gbv[RPT];
+        bool pv[RPT];
+        float acc[RPT], cmp[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            prow[j] = j * kWG + tid;
+            pcol[j] = 0;
+            lq[j] = prow[j];
+            acc[j] = 0.0f;
+            cmp[j] = 0.0f;
+            pv[j] = u0 + prow[j] < A.M;
+        }
+        // FAST (one shape, one offset): x[w0 + i], i < TRW, holds this tile's
+        // diagonal terms; it is loaded once, kept in LDS (s_win) and also
+        // serves the remainder gathers that fall inside it.
+        const int w0 = FAST ? u0 + A.sh0.ys - A.sh0.xs + A.sh0.omin : 0;
+        float xw[RPT];
+        if constexpr (FAST) {
+#pragma unroll
+            for (int j = 0; j < RPT; j++) xw[j] = load_x<XT>(X, (size_t)min(max(w0 + prow[j], 0), A.N - 1) * ldx);
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            const int rc = pv[j] ? u0 + prow[j] : A.M - 1;
+            // the wavefront's 64 rows are one group: its base is a scalar load
+            const int g = __builtin_amdgcn_readfirstlane(min((u0 + j * kWG + (tid & ~63)) >> 6, (A.M - 1) >> 6));
+            gbv[j] = cload(A.gbase, g);
+            endv[j] = OW == 1 ? (int)ro8[rc] : (int)ro16[rc];
+        }
+        const int ne0 = min(capv, E);
+        int cv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) cv[u] = u * kWG < ne0 ? A.cols[ebase + min(u * kWG + tid, ne0 - 1)] : 1;
+
+        if constexpr (FAST) {
+            const ShapeDev &sh = A.sh0;
+#pragma unroll
+            for (int j = 0; j < RPT; j++) {
+                s_win[prow[j]] = xw[j];
+                const int r = u0 + prow[j], c = w0 + prow[j];
+                if (pv[j] && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs))
+                    cadd(acc[j], cmp[j], xw[j]);
+            }
+            __syncthreads();
+        } else {
+            diag_phase<1, XT, RPT>(A, X, ldx, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+        }
+
+        // a row starts where its left neighbour ends (DPP wave_shr:1); lane 0
+        // starts the group
+        int rs[RPT], re[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            const int beg = __builtin_amdgcn_update_dpp(0, endv[j], 0x138, 0xF, 0xF, false);
+            rs[j] = gbv[j] - ebase + (lane == 0 ? 0 : beg);
+            re[j] = pv[j] ? gbv[j] - ebase + endv[j] : rs[j];
+        }
+
+        for (int e0 = 0; e0 < E; e0 += capv) {
+            const int ne = min(capv, E - e0);
+            for (int b = 0; b < ne; b += kWG * U) {
+                int vv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    vv[u] = 1;
+                    if (e0 == 0 && b == 0) vv[u] = cv[u];
+                    else if (b + u * kWG < ne) vv[u] = A.cols[ebase + e0 + min(b + u * kWG + tid, ne - 1)];
+                }
+                float xv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    xv[u] = 0.0f;
+                    if (b + u * kWG < ne) {
+                        const int col = vv[u] >> 1;
+                        if (FAST && (unsigned)(col - w0) < (unsigned)TRW) xv[u] = s_win[col - w0];
+                        else xv[u] = load_x<XT>(X, (size_t)col * ldx);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int idx = b + u * kWG + tid;
+                    if (idx < ne) s_val[idx] = (vv[u] & 1) ? xv[u] : -xv[u];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < RPT; j++) {
+                const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
+                float a = acc[j], e = cmp[j];
+                for (int k = lo; k < hi; k++) cadd(a, e, s_val[k - e0]);
+                acc[j] = a;
+                cmp[j] = e;
+            }
+            __syncthreads();
+        }
+
+        if constexpr (LONG) long_phase<1, XT, RPT>(A, X, ldx, u0, rows, lq, pv, acc, cmp, s_lsum, TRW);
+
+#pragma unroll
+        for (int j = 0; j < RPT; j++) {
+            if (!pv[j]) continue;
+            Y[(size_t)(u0 + prow[j]) * ldy] = acc[j] + cmp[j];
+        }
+    }
+}
+
+template <typename XT, int Q>
+__device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
+{
+    if constexpr (sizeof(XT) == 4) {
+        if constexpr (Q == 4) {
+            const float4 v = *(const float4 *)p;
+            o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w;
+        } else {
+            const float2 v = *(const float2 *)p;
+            o[0] = v.x, o[1] = v.y;
+        }
+    } else {
+        uint32_t w[Q / 2];
+        if constexpr (Q == 4) {
+            const uint2 v = *(const uint2 *)p;
+            w[0] = v.x, w[1] = v.y;
+        } else {
+            w[0] = *(const uint32_t *)p;
+        }
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            const uint16_t h = (uint16_t)(w[i >> 1] >> (16 * (i & 1)));
+            if constexpr (sizeof(XT) == 2 && __is_same(XT, bf16_t)) o[i] = __uint_as_float((uint32_t)h << 16);
+            else o[i] = (float)__builtin_bit_cast(_Float16, h);
+        }
+    }
+}
+
+// ---- k > 1: wide rows ----
+// A row belongs to a group of LPR = K/CPL lanes; lane q of the group owns the
+// CPL = min(K, 4) consecutive columns q*CPL.., so each x row segment the row
+// touches (a diagonal term or a remainder entry) is one CPL-wide vector load
+// per lane and y leaves as one vector store per lane. Consecutive groups own
+// consecutive rows (y is written contiguously when ldy = K). Rows are
+// independent: no LDS staging; only long rows (one wavefront each, the fixed
+// butterfly) meet in LDS. Same canonical per-(row, column) order as every
+// other kernel. Requires X, Y, ldx and ldy aligned to CPL elements.
+// LDS: the x window [(TRW + kWideHalo) * K], reused for the long-row sums.
+constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
+
+template <int K, typename XT, int RI, bool LONG>
+__device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
+                                          int ldy, int t, float *smem)
+{
+    constexpr int CPL = K < 4 ? K : 4;
+    constexpr int LPR = K / CPL;
+    constexpr int GPW = kWG / LPR;  // row groups per workgroup
+    constexpr int TRW = GPW * RI;   // rows per tile
+    constexpr int P = CPL * RI;     // (row, column) pairs per lane
+    constexpr int U = 4;            // remainder entries in flight per lane
+    const int tid = threadIdx.x;
+    const int g = tid / LPR, c0 = (tid % LPR) * CPL;
+    const uint8_t *ro8 = (const uint8_t *)A.roff;
+    const uint16_t *ro16 = (const uint16_t *)A.roff;
+    {
+        const int u0 = t * TRW;
+        float acc[P], cmp[P];
+        int lq[P];
+        bool pv[P];
+        int rs[RI], re[RI];
+#pragma unroll
+        for (int i = 0; i < RI; i++) {
+            const int r = u0 + GPW * i + g;
+            const bool valid = r < A.M;
+            const int rr = valid ? r : A.M - 1;
+            const int gb = A.gbase[rr >> 6];
+            const int prev = (rr & 63) ? (A.ow == 1 ? (int)ro8[rr - 1] : (int)ro16[rr - 1]) : 0;
+            const int end = A.ow == 1 ? (int)ro8[rr] : (int)ro16[rr];
+            rs[i] = gb + prev;
+            re[i] = valid ? gb + end : rs[i];
+#pragma unroll
+            for (int m = 0; m < CPL; m++) {
+                acc[i * CPL + m] = 0.0f;
+                cmp[i * CPL + m] = 0.0f;
+                lq[i * CPL + m] = (GPW * i + g) * K + c0 + m;
+                pv[i * CPL + m] = valid;
+            }
+        }
+
+        // diagonal terms: shapes in order, offsets in order; the rows' loads
+        // of one offset are issued together. A shape with two or more
+        // offsets first stages the tile's x window (rows [w0, w1), K columns)
+        // in LDS, so each x row segment is loaded once per tile, not once
+        // per offset.
+        for (int s = 0; s < A.S; s++) {
+            const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
+            if (u0 + TRW <= sh.xs || u0 >= sh.xe) continue;
+            const int side = sh.xe - sh.xs;
+            const int shift = sh.ys - sh.xs;
+            const int ra = max(u0, sh.xs), rb = min(min(u0 + TRW, sh.xe), A.M);
+            const int w0 = max(sh.ys, ra + shift + sh.omin);
+            const int w1 = min(sh.ys + side, rb - 1 + shift + sh.omax + 1);
+            if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
+                const int nv = (w1 - w0) * LPR;  // CPL-wide vectors of the window
+                for (int b = 0; b < nv; b += kWG * 4) {
+                    float tv[4][CPL];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int v = min(b + u * kWG + tid, nv - 1);
+                        load_q<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL, tv[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int v = b + u * kWG + tid;
+                        if (v < nv) {
+                            float *d = smem + (v / LPR) * K + (v % LPR) * CPL;
+                            if constexpr (CPL == 4) *(float4 *)d = make_float4(tv[u][0], tv[u][1], tv[u][2], tv[u][3]);
+                            else *(float2 *)d = make_float2(tv[u][0], tv[u][1]);
+                        }
+                    }
+                }
+                __syncthreads();
+                for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                    const int n1 = min(n0 + kDiagBlock, sh.oe);
+                    float bs[P];
+#pragma unroll
+                    for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                    for (int n = n0; n < n1; n++) {
+                        const int o = cload(A.offs, n);
+#pragma unroll
+                        for (int i = 0; i < RI; i++) {
+                            const int r = u0 + GPW * i + g;
+                            const int c = r + shift + o;
+                            if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
+                                const float *sp = smem + (c - w0) * K + c0;
+                                if constexpr (CPL == 4) {
+                                    const float4 q = *(const float4 *)sp;
+                                    bs[i * 4] += q.x, bs[i * 4 + 1] += q.y, bs[i * 4 + 2] += q.z, bs[i * 4 + 3] += q.w;
+                                } else {
+                                    const float2 q = *(const float2 *)sp;
+                                    bs[i * 2] += q.x, bs[i * 2 + 1] += q.y;
+                                }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+                }
+                __syncthreads();
+                continue;
+            }
+            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
+                const int n1 = min(n0 + kDiagBlock, sh.oe);
+                float bs[P];
+#pragma unroll
+                for (int p = 0; p < P; p++) bs[p] = 0.0f;
+                for (int n = n0; n < n1; n++) {
+                    const int o = cload(A.offs, n);
+                    float xv[RI][CPL];
+                    bool in[RI];
+#pragma unroll
+                    for (int i = 0; i < RI; i++) {
+                        const int r = u0 + GPW * i + g;
+                        const int c = r - sh.xs + sh.ys + o;
+                        in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
+                        if (in[i]) load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                    }
+#pragma unroll
+                    for (int i = 0; i < RI; i++)
+                        if (in[i])
+#pragma unroll
+                            for (int m = 0; m < CPL; m++) bs[i * CPL + m] += xv[i][m];
+                }
+#pragma unroll
+                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+            }
+        }
+
+        // remainder entries, U in flight, added in stored order
+#pragma unroll
+        for (int i = 0; i < RI; i++) {
+            for (int j0 = rs[i]; j0 < re[i]; j0 += U) {
+                int v[U];
+                float xv[U][CPL];
+#pragma unroll
+                for (int u = 0; u < U; u++) v[u] = j0 + u < re[i] ? A.cols[j0 + u] : 1;
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j0 + u < re[i]) load_q<XT, CPL>(X + (size_t)(v[u] >> 1) * ldx + c0, xv[u]);
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (j0 + u < re[i])
+#pragma unroll
+                        for (int m = 0; m < CPL; m++)
+                            cadd(acc[i * CPL + m], cmp[i * CPL + m], (v[u] & 1) ? xv[u][m] : -xv[u][m]);
+            }
+        }
+
+        if constexpr (LONG) long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, cmp, smem, TRW * K);
+
+#pragma unroll
+        for (int i = 0; i < RI; i++) {
+            const int r = u0 + GPW * i + g;
+            if (r >= A.M) continue;
+            float *yp = Y + (size_t)r * ldy + c0;
+            if constexpr (CPL == 4)
+                *(float4 *)yp = make_float4(acc[i * 4] + cmp[i * 4], acc[i * 4 + 1] + cmp[i * 4 + 1],
+                                            acc[i * 4 + 2] + cmp[i * 4 + 2], acc[i * 4 + 3] + cmp[i * 4 + 3]);
+            else *(float2 *)yp = make_float2(acc[i * 2] + cmp[i * 2], acc[i * 2 + 1] + cmp[i * 2 + 1]);
+        }
+    }
+}
+
+// LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
+__host__ __device__ constexpr size_t lds_bytes(int capv, int P, int trw)
+{
+    return ((size_t)capv + (size_t)kWG * P + (size_t)trw + 1) * 4;
+}
+
+}  // namespace tcd

@@ -1,0 +1,72 @@
+// The wide-rows kernel for k > 1 columns (a lane group per row, vector x
+// row segments) and its launcher.
+#include "spmv_host.h"
+
+namespace tcd {
+
+template <int K, typename XT, int RI, bool LONG>
+__global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
+                                                        float *__restrict__ Y, int ldy, int ntiles)
+{
+    extern __shared__ float smem[];
+    int tb, te;
+    block_units(ntiles, &tb, &te);
+    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG>(A, X, ldx, Y, ldy, t, smem);
+}
+
+template <int K, typename XT, int RI>
+int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int grid, hipStream_t s)
+{
+    constexpr int TRW = (kWG / (K / (K < 4 ? K : 4))) * RI;
+    const int ntiles = (A.M + TRW - 1) / TRW;
+    if (ntiles == 0) return TC_OK;
+    const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
+    const size_t lds = (A.S > 0 || A.L > 0) ? (size_t)(TRW + kWideHalo) * K * 4 : 0;
+    if (A.L > 0)
+        spmv_wide_kernel<K, XT, RI, true><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
+    else
+        spmv_wide_kernel<K, XT, RI, false><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
+    HIPCHECK(hipGetLastError());
+    return TC_OK;
+}
+
+template <int K, typename XT>
+int launch_wide_k(const MatView &A, const void *X, int ldx, float *Y, int ldy, const tc_config &c, hipStream_t s)
+{
+    if (c.rows_per_tile == wide_rows(K, 1)) return launch_wide_t<K, XT, 1>(A, X, ldx, Y, ldy, c.grid, s);
+    return launch_wide_t<K, XT, 2>(A, X, ldx, Y, ldy, c.grid, s);
+}
+
+template <typename XT>
+int launch_wide_x(const MatView &A, const void *X, int ldx, float *Y, int ldy, int K, const tc_config &c,
+                hipStream_t s)
+{
+    switch (K) {
+    case 2: return launch_wide_k<2, XT>(A, X, ldx, Y, ldy, c, s);
+    case 4: return launch_wide_k<4, XT>(A, X, ldx, Y, ldy, c, s);
+    case 8: return launch_wide_k<8, XT>(A, X, ldx, Y, ldy, c, s);
+    case 16: return launch_wide_k<16, XT>(A, X, ldx, Y, ldy, c, s);
+    case 32: return launch_wide_k<32, XT>(A, X, ldx, Y, ldy, c, s);
+    default: return launch_wide_k<64, XT>(A, X, ldx, Y, ldy, c, s);
+    }
+}
+
+// wide rows apply when X and Y rows split into aligned CPL-element vectors
+bool wide_applies(int K, const void *X, int ldx, const float *Y, int ldy, int esize, const tc_config &c)
+{
+    if (!wide_k(K) || (c.flags & kStagedK)) return false;
+    if (c.rows_per_tile != wide_rows(K, 1) && c.rows_per_tile != wide_rows(K, 2)) return false;
+    const int cpl = std::min(K, 4);
+    return ldx % cpl == 0 && ldy % cpl == 0 && (uintptr_t)X % ((uintptr_t)cpl * esize) == 0 &&
+           (uintptr_t)Y % ((uintptr_t)cpl * 4) == 0;
+}
+
+int launch_wide(const MatView &A, const void *X, int ldx, float *Y, int ldy, int dtype, int K, const tc_config &c,
+                hipStream_t s)
+{
+    if (dtype == TC_BF16) return launch_wide_x<bf16_t>(A, X, ldx, Y, ldy, K, c, s);
+    if (dtype == TC_F16) return launch_wide_x<f16_t>(A, X, ldx, Y, ldy, K, c, s);
+    return launch_wide_x<float>(A, X, ldx, Y, ldy, K, c, s);
+}
+
+}  // namespace tcd
