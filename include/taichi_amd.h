@@ -127,6 +127,17 @@ TaiChi_new *tc_taichi_new_alloc(void);
    and the struct itself */
 int tc_free_taichi_new(TaiChi_new *t);
 
+/* ---- format cache (no reference counterpart; the reference re-reads the
+   .mtx text and re-partitions on every run, TaiChi.cu:120-278, 313-638) ----
+   tc_save_format writes a partitioned TaiChi_new with its sizes and an
+   FNV-1a checksum; tc_load_format validates magic, sizes, checksum and the
+   row pointers, then fills *t with malloc'd arrays: free them with
+   tc_free_taichi_new on a heap struct (tc_taichi_new_alloc) -- which, like
+   free_taichi_new, leaves csrRow/csrCol alone when dense_nnz == 0: free
+   those two with tc_free_host. Bad files return TC_ERR_INVALID. */
+int tc_save_format(const char *path, int M, int N, const TaiChi_new *t);
+int tc_load_format(const char *path, int *M, int *N, TaiChi_new *t);
+
 /* ---- a7: one-shot host SpMV (taichi_SpMV) ---- */
 /* uploads the format and x, runs `warmup` + `runs` launches (10 + 500 in the
    reference), downloads y; optional timings in ms (NULL to skip) */

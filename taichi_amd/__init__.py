@@ -127,6 +127,8 @@ def lib():
         "tc_partition_shapes": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, i32, vp, vp,
                                           P(TaiChiNew)]),
         "tc_taichi_new_alloc": (P(TaiChiNew), []),
+        "tc_save_format": (C.c_int, [C.c_char_p, C.c_int, C.c_int, P(TaiChiNew)]),
+        "tc_load_format": (C.c_int, [C.c_char_p, P(C.c_int), P(C.c_int), P(TaiChiNew)]),
         "tc_free_taichi_new": (C.c_int, [P(TaiChiNew)]),
         "tc_taichi_spmv": (C.c_int, [C.c_int, C.c_int, P(TaiChiNew), f32, f32, C.c_int, C.c_int,
                                      P(C.c_float), P(C.c_float)]),
@@ -377,6 +379,30 @@ def _own(M, N, st):
         heap = L.tc_taichi_new_alloc()
         C.memmove(heap, C.byref(st), C.sizeof(TaiChiNew))
         L.tc_free_taichi_new(heap)
+    return fmt
+
+
+def save_format(path, fmt):
+    """Write a partitioned format to the on-disk cache (tc_save_format)."""
+    st = fmt.struct()
+    check(lib().tc_save_format(os.fsencode(path), fmt.M, fmt.N, C.byref(st)), "save_format")
+
+
+def load_format(path):
+    """Read a format written by save_format (validated); returns a Format."""
+    L = lib()
+    M, N = C.c_int(), C.c_int()
+    st = TaiChiNew()
+    check(L.tc_load_format(os.fsencode(path), C.byref(M), C.byref(N), C.byref(st)), "load_format")
+    fmt = Format(M.value, N.value, st)
+    heap = L.tc_taichi_new_alloc()
+    C.memmove(heap, C.byref(st), C.sizeof(TaiChiNew))
+    if st.dense_nnz == 0:  # free_taichi_new leaves the (here loaded) CSR alone
+        for f in ("csrRow", "csrCol"):
+            p = getattr(st, f)
+            if p:
+                L.tc_free_host(C.cast(p, C.c_void_p))
+    L.tc_free_taichi_new(heap)
     return fmt
 
 

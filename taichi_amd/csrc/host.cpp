@@ -152,6 +152,108 @@ int parse_banner(const char *line, char tc[4])
     return 0;
 }
 
+// Whole-line integer of the data section: optional sign, digits.
+inline bool line_int(const char *&q, const char *e, long long *v)
+{
+    while (q < e && (*q == ' ' || *q == '\t')) ++q;
+    bool neg = false;
+    if (q < e && (*q == '-' || *q == '+')) neg = *q++ == '-';
+    if (q >= e || *q < '0' || *q > '9') return false;
+    long long r = 0;
+    while (q < e && *q >= '0' && *q <= '9') {
+        if (r > (INT64_MAX - 9) / 10) return false;
+        r = r * 10 + (*q++ - '0');
+    }
+    *v = neg ? -r : r;
+    return true;
+}
+
+// Parallel parse of a line-structured data section: every non-blank line of
+// [p, end) holds exactly one entry (row, col and, unless pattern, a value).
+// Each thread parses a newline-aligned chunk; the chunks are joined in file
+// order and the first NZ entries kept, as the serial reader would read them.
+// Returns false -- and the caller parses serially, which gives the reference
+// semantics and error codes for any layout -- on a short file, an entry out
+// of range, fewer than NZ entries, or any line that is not exactly one entry.
+bool parse_lines_parallel(const char *p, const char *end, long long M, long long N, long long NZ, char kind,
+                          std::vector<int> &ri, std::vector<int> &ci, std::vector<float> &vi)
+{
+    if (end - p < (1 << 22)) return false;  // small files: the serial parse is as fast
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<const char *> bnd((size_t)T + 1);
+    bnd[0] = p;
+    bnd[T] = end;
+    for (int t = 1; t < T; t++) {
+        const char *q = p + (end - p) * t / T;
+        if (q < bnd[t - 1]) q = bnd[t - 1];
+        const char *nl = (const char *)memchr(q, '\n', end - q);
+        bnd[t] = nl ? nl + 1 : end;
+    }
+    struct Part {
+        std::vector<int> r, c;
+        std::vector<float> v;
+        bool ok = true;
+    };
+    std::vector<Part> parts((size_t)T);
+    tc::parallel_for(T, [&](long long tb, long long te) {
+        for (long long t = tb; t < te; t++) {
+            Part &pt = parts[t];
+            const char *q = bnd[t], *e = bnd[t + 1];
+            pt.r.reserve((size_t)((e - q) / 8));
+            pt.c.reserve((size_t)((e - q) / 8));
+            while (q < e && pt.ok) {
+                const char *le = (const char *)memchr(q, '\n', e - q);
+                if (!le) le = e;
+                const char *x = q;
+                while (x < le && isspace((unsigned char)*x)) ++x;
+                if (x < le) {
+                    long long a, b, iv;
+                    double fv = 1.0;
+                    bool good = line_int(x, le, &a) && line_int(x, le, &b);
+                    if (good && kind == 'I') {
+                        good = line_int(x, le, &iv);
+                        fv = (double)(int)iv;
+                    } else if (good && kind == 'R') {
+                        char *y;
+                        fv = strtod(x, &y);
+                        good = y != x && y <= le;
+                        x = y;
+                    }
+                    while (good && x < le && isspace((unsigned char)*x)) ++x;
+                    good = good && x == le && a >= 1 && a <= M && b >= 1 && b <= N;
+                    if (!good) {
+                        pt.ok = false;
+                        break;
+                    }
+                    pt.r.push_back((int)(a - 1));
+                    pt.c.push_back((int)(b - 1));
+                    if (kind != 'P') pt.v.push_back((float)fv);
+                }
+                q = le + 1;
+            }
+        }
+    }, T);
+    long long total = 0;
+    for (const Part &pt : parts) {
+        if (!pt.ok) return false;
+        total += (long long)pt.r.size();
+    }
+    if (total < NZ) return false;
+    ri.resize((size_t)NZ);
+    ci.resize((size_t)NZ);
+    vi.assign((size_t)NZ, 1.0f);
+    long long o = 0;
+    for (const Part &pt : parts) {
+        const long long take = std::min<long long>((long long)pt.r.size(), NZ - o);
+        if (take <= 0) break;
+        std::copy(pt.r.begin(), pt.r.begin() + take, ri.begin() + o);
+        std::copy(pt.c.begin(), pt.c.begin() + take, ci.begin() + o);
+        if (kind != 'P') std::copy(pt.v.begin(), pt.v.begin() + take, vi.begin() + o);
+        o += take;
+    }
+    return true;
+}
+
 }  // namespace
 
 extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA, int **rowptr,
@@ -195,10 +297,21 @@ extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA
         return tc::fail(-4, "unsupported size line or storage");
     const bool isReal = tc[2] == 'R', isInt = tc[2] == 'I';
     const bool isSym = tc[3] == 'S' || tc[3] == 'H';
-    std::vector<int> ri((size_t)NZ), ci((size_t)NZ);
-    std::vector<float> vi((size_t)NZ);
+    std::vector<int> ri, ci;
+    std::vector<float> vi;
     std::vector<long long> cnt((size_t)M + 1, 0);
-    for (long long i = 0; i < NZ; i++) {
+    const bool parallel = parse_lines_parallel(cur.p, end, M, N, NZ, tc[2], ri, ci, vi);
+    if (parallel) {
+        for (long long i = 0; i < NZ; i++) {
+            cnt[ri[i]]++;
+            if (isSym && ri[i] != ci[i]) cnt[ci[i]]++;
+        }
+    } else {
+        ri.resize((size_t)NZ);
+        ci.resize((size_t)NZ);
+        vi.resize((size_t)NZ);
+    }
+    for (long long i = 0; i < NZ && !parallel; i++) {
         long long a, b;
         double fv = 1.0;
         if (!cur.read_long(&a) || !cur.read_long(&b))
