@@ -101,6 +101,11 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                            "(the HIP library has no fallback)")
+    # torch first: its HIP runtime (soname libamdhip64.so.7) then satisfies
+    # the library's dependency, so the library and torch's tensors share one
+    # runtime. Loaded the other way round, torch maps a second HIP/HSA runtime
+    # beside /opt/rocm's and the library's runtime finds no device.
+    import torch  # noqa: F401
     L = C.CDLL(LIB_PATH)
     P = C.POINTER
     i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -146,6 +151,9 @@ def lib():
         "tc_graph_capture": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int,
                                        vp, P(vp)]),
         "tc_graph_launch": (C.c_int, [vp, vp]),
+        "tc_pagerank_work_bytes": (C.c_size_t, [C.c_int]),
+        "tc_pagerank": (C.c_int, [vp, vp, C.c_float, C.c_int, vp, vp, vp]),
+        "tc_pagerank_capture": (C.c_int, [vp, vp, C.c_float, C.c_int, vp, vp, vp, P(vp)]),
         "tc_graph_destroy": (C.c_int, [vp]),
         "tc_matrix_footprint": (C.c_int, [C.c_int, C.c_int, P(TaiChiNew), C.c_int,
                                           P(C.c_longlong)]),
@@ -380,6 +388,43 @@ def _own(M, N, st):
         C.memmove(heap, C.byref(st), C.sizeof(TaiChiNew))
         L.tc_free_taichi_new(heap)
     return fmt
+
+
+def inverse_out_degree(N, col):
+    """inv_deg[j] = 1 / (entries in column j), 0 for an empty column: the
+    PageRank scaling of a binary A where A[i][j] = 1 means j links to i."""
+    deg = np.bincount(np.asarray(col, np.int64), minlength=N).astype(np.float64)
+    inv = np.zeros(N, np.float32)
+    nz = deg > 0
+    inv[nz] = (1.0 / deg[nz]).astype(np.float32)
+    return inv
+
+
+class PageRank:
+    """PageRank power iteration on a resident square Matrix (tc_pagerank):
+    inv_deg is a device float32 tensor (inverse_out_degree); run() enqueues
+    the iterations, capture() records them once as a replayable graph."""
+
+    def __init__(self, A, inv_deg, alpha=0.85):
+        import torch
+        self.A, self.inv_deg, self.alpha = A, inv_deg, float(alpha)
+        n = inv_deg.shape[0]
+        self.work = torch.empty(int(lib().tc_pagerank_work_bytes(n)), dtype=torch.uint8,
+                                device=inv_deg.device)
+
+    def run(self, r, iters, stream=None):
+        check(lib().tc_pagerank(self.A.handle, self.inv_deg.data_ptr(), self.alpha, iters,
+                                r.data_ptr(), self.work.data_ptr(), _torch_stream(stream)),
+              "pagerank")
+
+    def capture(self, r, iters, stream=None):
+        h = C.c_void_p()
+        check(lib().tc_pagerank_capture(self.A.handle, self.inv_deg.data_ptr(), self.alpha, iters,
+                                        r.data_ptr(), self.work.data_ptr(), stream, C.byref(h)),
+              "pagerank_capture")
+        g = Graph.__new__(Graph)
+        g.handle = h
+        return g
 
 
 def save_format(path, fmt):

@@ -2,6 +2,7 @@
 function that include/taichi_amd.h and include/TaiChi.h declare (no compute
 call is made). Also checks the generators' determinism, which the GPU tests
 and the benchmark rely on."""
+import ast
 import ctypes as C
 import os
 import re
@@ -82,3 +83,17 @@ def test_generators_are_deterministic():
     assert set(np.unique(xi).tolist()) <= set(range(8))
     xu = T.gen_vector(10000, 0, 5)
     assert xu.min() >= 0.0 and xu.max() < 1.0
+
+
+def test_library_shares_torchs_hip_runtime():
+    """Loading the library before torch must not leave two HIP runtimes in the
+    process (torch would map its own beside /opt/rocm's, and the library's
+    would then find no device)."""
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import taichi_amd as T\nT.lib()\nimport torch\n"
+            "print(sorted({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}))"
+            % ROOT)
+    out = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=300, cwd="/tmp")
+    assert out.returncode == 0, out.stderr
+    maps = ast.literal_eval(out.stdout.strip().splitlines()[-1])
+    assert len(maps) == 1, maps
