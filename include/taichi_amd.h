@@ -283,6 +283,22 @@ int tc_graph_capture(tc_matrix *A, const void *X, int ldx, int dtype, float *Y,
 int tc_graph_launch(tc_graph *g, void *stream);
 int tc_graph_destroy(tc_graph *g);
 
+/* ---- iterative driver (no reference counterpart; the step after SpMV) ----
+   PageRank by power iteration on a resident square binary matrix: A[i][j] =
+   1 means j links to i; inv_deg[j] = 1/outdeg(j) or 0 for a page without
+   out-links, whose rank is spread uniformly. From r = 1/n, `iters` steps of
+   r <- (1 - alpha)/n + alpha (A (r .* inv_deg) + dangling/n), all on the
+   device (each step: tc_spmv, a fused update, a fixed-order reduction).
+   inv_deg, r (n floats, written) and work (tc_pagerank_work_bytes(n)) are
+   device buffers. tc_pagerank enqueues on `stream`; tc_pagerank_capture
+   records the same sequence once as a graph for tc_graph_launch (replays
+   give identical ranks). */
+size_t tc_pagerank_work_bytes(int n);
+int tc_pagerank(tc_matrix *A, const float *inv_deg, float alpha, int iters, float *r, void *work,
+                void *stream);
+int tc_pagerank_capture(tc_matrix *A, const float *inv_deg, float alpha, int iters, float *r,
+                        void *work, void *stream, tc_graph **out);
+
 /* ---- synthetic generators (no reference counterpart; the reference reads
    downloaded datasets, README.md:17-20). Deterministic for a seed on any
    thread count. CSR arrays are malloc'd (tc_free_host); rows sorted, no

@@ -10,10 +10,6 @@
 #include "batch.h"
 #include "spmv_host.h"
 
-struct tc_graph {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
 
 namespace tcd {
 
@@ -246,9 +242,10 @@ int tc_matrix_create(int M, int N, const TaiChi_new *T, long long nnz, int long_
         fmt_bytes += p.bytes;
     }
     tc_matrix *A = new tc_matrix();
-    if (hipGetDevice(&A->device) != hipSuccess) {
+    const hipError_t de = hipGetDevice(&A->device);
+    if (de != hipSuccess) {
         delete A;
-        return tc::fail(TC_ERR_NODEVICE, "tc_matrix_create: no HIP device");
+        return tc::fail(TC_ERR_NODEVICE, "tc_matrix_create: no HIP device (hipGetDevice: %s)", hipGetErrorString(de));
     }
     hipStream_t s = (hipStream_t)stream;
     if (total > 0) {

@@ -29,6 +29,12 @@ struct tc_matrix {
     std::map<std::pair<int, int>, tc_config> cfg;
 };
 
+// a captured launch sequence (tc_graph_capture, tc_pagerank_capture)
+struct tc_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 namespace tcd {
 
 inline int elem_size(int dtype) { return dtype == TC_F32 ? 4 : 2; }
