@@ -15,6 +15,7 @@
  *   tc_sdf_from_image ...... getSdf morphology  morohutils.cu:231-282
  *   tc_partition_from_image  partition_shapes   TaiChi.h:83-84 (given the image)
  *   tc_partition_shapes .... partition_shapes   TaiChi.h:83-84, TaiChi.cu:313-638
+ *   tc_partition_device .... partition_dia_shape's passes, TaiChi.cu:677-879 (device)
  *   tc_cal_shape_size ...... cal_shape_size     TaiChi.h:87-88, TaiChi.cu:280-311
  *   tc_cal_start_end ....... cal_start_end      TaiChi.h:103, TaiChi.cu:881-916
  *   tc_free_taichi_new ..... free_taichi_new    TaiChi.h:106, TaiChi.cu:77-118
@@ -117,10 +118,23 @@ void tc_cal_start_end(int M_sub, int N_sub, int width, int off, int n,
    reference's (FLT_MAX on claimed entries). out must be zeroed. */
 int tc_partition_from_image(int M, int N, int nnz, int *rowptr, int *col,
                             float *val, const unsigned char *img, TaiChi_new *out);
-/* partition_shapes: uploads the CSR, runs tc_thumbnail on the device, then
+/* partition_shapes: uploads the CSR and builds the format on the device
+   (tc_partition_device); with val given, or when the device build returns
+   TC_ERR_UNSUPPORTED, runs tc_thumbnail on the device and then
    tc_partition_from_image. path is accepted for signature parity (unused). */
 int tc_partition_shapes(const char *path, int M, int N, int nnz, int *rowptr,
                         int *col, float *val, TaiChi_new *out);
+/* Device build of partition_shapes for a binary CSR (the histogram, claim and
+   zero-row passes of partition_dia_shape, TaiChi.cu:723-734, 809-827, 850-871,
+   and the merge :480-612, as kernels). d_rowptr/d_col are device copies of
+   rowptr/col; the format's arrays come back in host memory, identical to
+   tc_partition_from_image's (with no dense diagonal they alias rowptr/col).
+   TC_ERR_UNSUPPORTED when an entry is duplicated on a dense diagonal (the
+   reference's zero-row walk then depends on the duplicate) or the format
+   would pass 2^31 entries. */
+int tc_partition_device(int M, int N, int nnz, int *rowptr, int *col,
+                        const int *d_rowptr, const int *d_col, TaiChi_new *out,
+                        void *stream);
 /* zeroed malloc'd struct, for callers that cannot malloc (FFI) */
 TaiChi_new *tc_taichi_new_alloc(void);
 /* free_taichi_new: frees the arrays (csrRow/csrCol only when dense_nnz > 0)

@@ -131,6 +131,8 @@ def lib():
                                               P(TaiChiNew)]),
         "tc_partition_shapes": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, i32, vp, vp,
                                           P(TaiChiNew)]),
+        "tc_partition_device": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, vp, vp, vp, P(TaiChiNew),
+                                          vp]),
         "tc_taichi_new_alloc": (P(TaiChiNew), []),
         "tc_save_format": (C.c_int, [C.c_char_p, C.c_int, C.c_int, P(TaiChiNew)]),
         "tc_load_format": (C.c_int, [C.c_char_p, P(C.c_int), P(C.c_int), P(TaiChiNew)]),
@@ -366,8 +368,23 @@ def partition_from_image(M, N, rowptr, col, val=None, img=None):
     return fmt, v
 
 
+def partition_device(M, N, rowptr, col, rowptr_dev, col_dev, stream=None):
+    """Device build of partition_shapes (tc_partition_device) for a binary CSR;
+    rowptr_dev/col_dev are int32 device tensors holding rowptr/col. Raises
+    TaiChiError (status -104) when an entry is duplicated on a dense diagonal."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    st = TaiChiNew()
+    check(lib().tc_partition_device(M, N, len(col), rowptr, _vp(col), rowptr_dev.data_ptr(),
+                                    col_dev.data_ptr() if len(col) else None, C.byref(st),
+                                    _torch_stream(stream)), "partition_device")
+    return _own(M, N, st)
+
+
 def partition_shapes(M, N, rowptr, col, val=None, path="matrix.mtx"):
-    """partition_shapes (TaiChi.cu:313-638): GPU thumbnail kernel + host partition."""
+    """partition_shapes (TaiChi.cu:313-638): device build (GPU thumbnail,
+    histogram, claim, zero rows and merge); with val, or duplicates on a dense
+    diagonal, GPU thumbnail + host partition."""
     rowptr = np.ascontiguousarray(rowptr, np.int32)
     col = np.ascontiguousarray(col, np.int32)
     v = None if val is None else np.array(val, np.float32, copy=True)

@@ -30,15 +30,6 @@ namespace {
 constexpr int kThumbH = 512;     // morohutils.cu:191
 constexpr int kMinShape = 1000;  // TaiChi.cu:379
 
-int width_tier(int sd)
-{
-    if (sd >= 10000000) return (int)(sd * 0.000001);
-    if (sd >= 1000000) return (int)(sd * 0.00001);
-    if (sd >= 100000) return (int)(sd * 0.001);
-    if (sd >= 10000) return (int)(sd * 0.01);
-    return (int)(sd * 0.1);
-}
-
 template <class T>
 T *dup_array(const std::vector<T> &v)
 {
@@ -48,6 +39,98 @@ T *dup_array(const std::vector<T> &v)
 }
 
 }  // namespace
+
+// diagonal band half-width of a square of this side (TaiChi.cu:689-699)
+int tc::width_tier(int sd)
+{
+    if (sd >= 10000000) return (int)(sd * 0.000001);
+    if (sd >= 1000000) return (int)(sd * 0.00001);
+    if (sd >= 100000) return (int)(sd * 0.001);
+    if (sd >= 10000) return (int)(sd * 0.01);
+    return (int)(sd * 0.1);
+}
+
+// Candidate squares in contour order (TaiChi.cu:360-381): each contour's
+// corners scaled by cal_shape_size, too-small ones dropped.
+int tc::candidate_squares(int M, int N, const unsigned char *img, std::vector<Square> &out)
+{
+    out.clear();
+    int H, W, pM, pN;
+    std::vector<int> pts;
+    if (tc_thumb_dims(M, N, &H, &W, &pM, &pN) == 0) {
+        if (!img) return tc::fail(TC_ERR_INVALID, "partition: thumbnail required for M > 512");
+        pts = tc::sdf_from_image(img, H, W);
+    }
+    const int num_shapes = (int)pts.size() / 4;
+    for (int i = 0; i < num_shapes; i++) {
+        int x1 = pts[4 * i], y1 = pts[4 * i + 1];
+        int a1 = pts[4 * i + 2] - x1, b1 = pts[4 * i + 3] - y1;  // TaiChi.cu:368-371
+        int x, y, a, b;
+        float gain;
+        tc_cal_shape_size(M, N, kThumbH, x1, y1, a1, b1, &x, &y, &a, &b, &gain);
+        if (a < kMinShape || b < kMinShape) continue;
+        out.push_back({x, y, b});
+    }
+    return TC_OK;
+}
+
+// The density test of partition_dia_shape (TaiChi.cu:736-760) from the
+// per-diagonal counts of unclaimed entries in the square's band: fills offs,
+// st, en and nnz; dense_of[n] is the dense index of diagonal n, or -1.
+void tc::decide_dense(int side, int w, const long long *cnt, DiaShape &sh, std::vector<int> &dense_of)
+{
+    const int nd = 2 * w - 1;
+    dense_of.assign((size_t)(nd > 0 ? nd : 0), -1);
+    for (int n = 0; n < nd; n++) {
+        int off = n - w + 1, s, e;
+        tc_cal_start_end(side, side, w, off, n, &s, &e);
+        float density = (float)cnt[n] / (float)(e - s);
+        if ((double)density >= 0.6) {
+            dense_of[n] = (int)sh.offs.size();
+            sh.offs.push_back(off);
+            sh.st.push_back(s);
+            sh.en.push_back(e);
+            sh.nnz += cnt[n];
+        }
+    }
+}
+
+// Shape arrays and counters of the format (TaiChi.cu:423-481) from the kept
+// shapes; csrRow, csrCol and zero_elements are left to the caller.
+void tc::shape_arrays(const std::vector<DiaShape> &kept, int nnz, TaiChi_new *out)
+{
+    const int S = (int)kept.size();
+    long long D = 0, dense = 0;
+    for (auto &s : kept) {
+        D += (long long)s.offs.size();
+        dense += s.nnz;
+    }
+    out->dia_shapes = S;
+    out->num_negs = (int)D;
+    out->dense_nnz = (int)dense;
+    out->sparse_nnz = (int)(nnz - dense);
+    if (S == 0) return;
+    std::vector<int> xs, ys, xe, ptr{0}, offs;
+    for (auto &s : kept) {
+        xs.push_back(s.xs);
+        ys.push_back(s.ys);
+        xe.push_back(s.xs + s.side);
+        ptr.push_back(ptr.back() + (int)s.offs.size());
+        offs.insert(offs.end(), s.offs.begin(), s.offs.end());
+    }
+    out->xStart = dup_array(xs);
+    out->yStart = dup_array(ys);
+    out->xStop = dup_array(xe);
+    out->nDiasPtr = dup_array(ptr);
+    out->neg_offsets = dup_array(offs);
+    int lo = xs[0], hi = xe[0];
+    for (int i = 1; i < S; i++) {
+        lo = xs[i] < lo ? xs[i] : lo;
+        hi = xe[i] > hi ? xe[i] : hi;
+    }
+    out->row_start = lo;
+    out->row_stop = hi;
+}
 
 // One square shape [x, x+side) x [y, y+side) (partition_dia_shape,
 // TaiChi.cu:677-879).
@@ -75,20 +158,9 @@ void tc::partition_square(const int *rowptr, const int *col, std::vector<uint8_t
             int n = diag(r, col[j]);
             if (n >= 0) cnt[n]++;
         }
-    std::vector<int> dense_of((size_t)nd, -1);
-    std::vector<int> &st = sh.st, &en = sh.en;
-    for (int n = 0; n < nd; n++) {
-        int off = n - w + 1, s, e;
-        tc_cal_start_end(side, side, w, off, n, &s, &e);
-        float density = (float)cnt[n] / (float)(e - s);
-        if ((double)density >= 0.6) {
-            dense_of[n] = (int)sh.offs.size();
-            sh.offs.push_back(off);
-            st.push_back(s);
-            en.push_back(e);
-            sh.nnz += cnt[n];
-        }
-    }
+    std::vector<int> dense_of;
+    decide_dense(side, w, cnt.data(), sh, dense_of);
+    const std::vector<int> &st = sh.st, &en = sh.en;
     const int neg = (int)sh.offs.size();
     if (neg == 0) return;
     // rows holding a nonzero on each dense diagonal (ascending), claiming them
@@ -196,61 +268,23 @@ int tc_partition_from_image(int M, int N, int nnz, int *rowptr, int *col, float 
         }
     std::vector<uint8_t> pre = claimed;
 
-    int H, W, pM, pN;
-    std::vector<int> pts;
-    if (tc_thumb_dims(M, N, &H, &W, &pM, &pN) == 0) {
-        if (!img) return tc::fail(TC_ERR_INVALID, "tc_partition_from_image: thumbnail required for M > 512");
-        pts = tc::sdf_from_image(img, H, W);
-    }
-    const int num_shapes = (int)pts.size() / 4;
+    std::vector<tc::Square> cand;
+    int rc = tc::candidate_squares(M, N, img, cand);
+    if (rc != TC_OK) return rc;
     std::vector<tc::DiaShape> kept;
-    for (int i = 0; i < num_shapes; i++) {
-        int x1 = pts[4 * i], y1 = pts[4 * i + 1];
-        int a1 = pts[4 * i + 2] - x1, b1 = pts[4 * i + 3] - y1;  // TaiChi.cu:368-371
-        int x, y, a, b;
-        float gain;
-        tc_cal_shape_size(M, N, kThumbH, x1, y1, a1, b1, &x, &y, &a, &b, &gain);
-        if (a < kMinShape || b < kMinShape) continue;
+    for (const tc::Square &q : cand) {
         tc::DiaShape sh;
-        tc::partition_square(rowptr, col, claimed, x, y, b, sh);
+        tc::partition_square(rowptr, col, claimed, q.x, q.y, q.side, sh);
         if (sh.nnz > 0) kept.push_back(std::move(sh));
     }
 
     // ---- format (TaiChi.cu:423-612) ----
-    const int S = (int)kept.size();
-    long long D = 0, dense = 0, zeros = 0;
-    for (auto &s : kept) {
-        D += (long long)s.offs.size();
-        dense += s.nnz;
+    tc::shape_arrays(kept, nnz, out);
+    const long long dense = out->dense_nnz;
+    long long zeros = 0;
+    for (auto &s : kept)
         for (auto &z : s.zeros) zeros += (long long)z.size();
-    }
-    out->dia_shapes = S;
-    out->num_negs = (int)D;
-    out->dense_nnz = (int)dense;
     out->zero_elements = (int)zeros;
-    out->sparse_nnz = (int)(nnz - dense);
-    if (S > 0) {
-        std::vector<int> xs, ys, xe, ptr{0}, offs;
-        for (auto &s : kept) {
-            xs.push_back(s.xs);
-            ys.push_back(s.ys);
-            xe.push_back(s.xs + s.side);
-            ptr.push_back(ptr.back() + (int)s.offs.size());
-            offs.insert(offs.end(), s.offs.begin(), s.offs.end());
-        }
-        out->xStart = dup_array(xs);
-        out->yStart = dup_array(ys);
-        out->xStop = dup_array(xe);
-        out->nDiasPtr = dup_array(ptr);
-        out->neg_offsets = dup_array(offs);
-        int lo = xs[0], hi = xe[0];
-        for (int i = 1; i < S; i++) {
-            lo = xs[i] < lo ? xs[i] : lo;
-            hi = xe[i] > hi ? xe[i] : hi;
-        }
-        out->row_start = lo;
-        out->row_stop = hi;
-    }
     if (dense == 0) {  // no dense diagonal: the input CSR is the format (:575-578)
         out->csrRow = rowptr;
         out->csrCol = col;

@@ -5,6 +5,9 @@
 #include <cstdint>
 #include <vector>
 
+#include "TaiChi.h"
+
+
 namespace tc {
 
 // getSdf's image pipeline: x1,y1,x2,y2 per kept contour
@@ -22,5 +25,14 @@ struct DiaShape {
 };
 void partition_square(const int *rowptr, const int *col, std::vector<uint8_t> &claimed, int x, int y, int side,
                       DiaShape &sh);
+
+// Steps of the partition shared by the host and the device builds.
+struct Square {
+    int x, y, side;
+};
+int width_tier(int side);
+int candidate_squares(int M, int N, const unsigned char *img, std::vector<Square> &out);
+void decide_dense(int side, int w, const long long *cnt, DiaShape &sh, std::vector<int> &dense_of);
+void shape_arrays(const std::vector<DiaShape> &kept, int nnz, TaiChi_new *out);
 
 }  // namespace tc

@@ -142,12 +142,17 @@ extern "C" int tc_partition_shapes(const char *path, int M, int N, int nnz, int 
         HIPCHECK(hipMalloc((void **)&dimg, img.size()));
         HIPCHECK(hipMemcpy(drow, rowptr, ((size_t)M + 1) * sizeof(int), hipMemcpyHostToDevice));
         if (nnz) HIPCHECK(hipMemcpy(dcol, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
-        int rc = tc_thumbnail(M, N, drow, dcol, dimg, nullptr);
-        if (rc == TC_OK) HIPCHECK(hipMemcpy(img.data(), dimg, img.size(), hipMemcpyDeviceToHost));
+        int rc = TC_ERR_UNSUPPORTED;
+        if (!val) rc = tc_partition_device(M, N, nnz, rowptr, col, drow, dcol, out, nullptr);
+        const bool host_build = rc == TC_ERR_UNSUPPORTED;  // val given, or duplicates on a dense diagonal
+        if (host_build) {
+            rc = tc_thumbnail(M, N, drow, dcol, dimg, nullptr);
+            if (rc == TC_OK) HIPCHECK(hipMemcpy(img.data(), dimg, img.size(), hipMemcpyDeviceToHost));
+        }
         (void)hipFree(drow);
         (void)hipFree(dcol);
         (void)hipFree(dimg);
-        if (rc != TC_OK) return rc;
+        if (!host_build || rc != TC_OK) return rc;
     }
     return tc_partition_from_image(M, N, nnz, rowptr, col, val, img.empty() ? nullptr : img.data(), out);
 }
