@@ -222,7 +222,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the batched, served and multi-RHS sections")
+                    help="skip the pagerank, batched, served and multi-RHS sections")
+    ap.add_argument("--sections", default="pagerank,batched,served,multi_rhs",
+                    help="comma-separated extra sections to run")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--config", default="",
                     help="rows_per_tile,lds_values,grid[,flags] (skips the autotuner)")
@@ -371,18 +373,21 @@ def main():
         "max_rel_err_vs_fp64": err,
         "wall_s_timed": round(wall, 4),
     }
-    if not args.no_extra:
+    extra = set() if args.no_extra else set(args.sections.split(","))
+    if "pagerank" in extra:
         log("iterative driver: PageRank on the same matrix")
         out["pagerank"] = bench_pagerank(A, n, col, dev)
         log(f"  {out['pagerank']}")
     del A, xd, yd
-    if not args.no_extra:
+    if "batched" in extra:
         log("config 4: batched uneven matrices")
         out["batched"] = bench_batched(dev)
         log(f"  {out['batched']}")
+    if "served" in extra:
         log("config 5: served mix")
         out["served"] = bench_served(dev)
         log(f"  {out['served']}")
+    if "multi_rhs" in extra:
         log("config 3: multi-RHS")
         out["multi_rhs"] = bench_multirhs(dev)
         log(f"  {out['multi_rhs']}")

@@ -26,7 +26,7 @@ def _pagerank64(M, rp, col, alpha, iters):
     return r
 
 
-@pytest.mark.parametrize("case", ["rmat16", "banded"])
+@pytest.mark.parametrize("case", ["rmat16", "banded", "odd_n", "unaligned_r"])
 def test_pagerank_matches_fp64(case, dev):
     if case == "rmat16":  # pages j % 97 == 0 lose their out-links (dangling)
         M, N, rp, col = rmat(16)
@@ -35,15 +35,15 @@ def test_pagerank_matches_fp64(case, dev):
         col = col[keep].astype(np.int32)
         rp = np.zeros(M + 1, np.int32)
         rp[1:] = np.cumsum(np.bincount(rows, minlength=M))
-    else:
-        M = N = 40000
+    else:  # odd_n: n % 4 = 1 (vector tail); unaligned_r: r 4 bytes off (scalar path)
+        M = N = 40001 if case == "odd_n" else 40000
         rp, col = T.gen_banded(M, 9, 0.7, 13)
     fmt = host_format(M, N, rp, col)
     A = T.Matrix(fmt)
     inv = torch.from_numpy(T.inverse_out_degree(N, col)).to(dev)
     pr = T.PageRank(A, inv, alpha=0.85)
     iters = 30
-    r = torch.empty(M, device=dev)
+    r = torch.empty(M + 1, device=dev)[1:] if case == "unaligned_r" else torch.empty(M, device=dev)
     pr.run(r, iters)
     torch.cuda.synchronize()
     want = _pagerank64(M, rp, col, 0.85, iters)
