@@ -54,6 +54,20 @@ def read_traffic(name):
     return None
 
 
+def _rel_err_fp64(rp, col, ncols, X, Y):
+    """max over entries of |Y - A X| / max(|A X|, A |X|), with A X the FP64
+    product of the binary CSR (scipy): the componentwise error bound of an
+    FP32 sum of the same terms."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix((np.ones(len(col)), np.asarray(col, np.int64), np.asarray(rp, np.int64)),
+                      shape=(len(rp) - 1, ncols))
+    X = np.asarray(X, np.float64)
+    ref = A @ X
+    den = np.maximum(np.abs(ref), A @ np.abs(X))
+    den[den == 0] = 1.0
+    return float(np.max(np.abs(np.asarray(Y, np.float64) - ref) / den)) if ref.size else 0.0
+
+
 def _time_events(fn, reps, stream):
     import torch
     e0 = torch.cuda.Event(enable_timing=True)
@@ -150,10 +164,9 @@ def bench_served(dev, requests=40000, rate=20000.0):
 
 def bench_multirhs(dev, ks=(1, 4, 16, 64)):
     """2048^2 5-point stencil + 2% uniform ones (seed 2): k right-hand sides
-    N(0,1), FP32 and BF16 storage of X, FP32 accumulation; error against the
-    FP64 oracle on the quantised X, componentwise."""
+    N(0,1), FP32 and BF16 storage of X, FP32 accumulation; error against an
+    FP64 product on the quantised X, componentwise, over a row sample."""
     import torch
-    import oracle
     import taichi_amd as T
     from taichi_amd import workloads as W
     M, N, rp, col = W.stencil5()
@@ -172,20 +185,85 @@ def bench_multirhs(dev, ks=(1, 4, 16, 64)):
             X = Xf.to(dt)
             ms = _time_events(lambda: A.spmv(X, Y), 10, stream)
             Xq = X.float().cpu().numpy().astype(np.float64)
-            Y64 = oracle.csr_spmm_f64(sub_rp, sub_col, Xq)
-            absY = oracle.csr_spmm_f64(sub_rp, sub_col, np.abs(Xq))
-            Ys = Y.cpu().numpy()[sample].astype(np.float64)
-            den = np.maximum(np.abs(Y64), absY)
-            den[den == 0] = 1
-            err = float(np.max(np.abs(Ys - Y64) / den))
-            Y64u = oracle.csr_spmm_f64(sub_rp, sub_col, Xf.cpu().numpy().astype(np.float64))
-            err_u = float(np.max(np.abs(Ys - Y64u) / den))
+            Ys = Y.cpu().numpy()[sample]
+            err = _rel_err_fp64(sub_rp, sub_col, N, Xq, Ys)
+            err_u = _rel_err_fp64(sub_rp, sub_col, N, Xf.cpu().numpy().astype(np.float64), Ys)
             B = A.bytes(k, T.BF16 if name == "bf16" else T.F32)
             out["runs"].append({"k": k, "x": name, "ms": round(ms, 4),
                                 "gnnz_per_s": round(len(col) * k / ms / 1e6, 2),
                                 "gbs": round(B / ms / 1e6, 1), "rel_err_vs_fp64": err,
                                 "rel_err_vs_fp64_unquantised_x": err_u})
     return out
+
+
+def _wall(fn, reps=3):
+    """median wall time of fn() over reps runs, device synchronised"""
+    import torch
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def bench_partition(n, rp, col, dev):
+    """Partition + format build of the headline matrix: the device build
+    (tc_partition_device on resident rowptr/col, format copied back), the
+    whole partition_shapes call (CSR upload included), and the host build
+    given the same thumbnail. Device and host formats must be identical."""
+    import ctypes as C
+    import torch
+    import taichi_amd as T
+    rpd = torch.from_numpy(rp).to(dev)
+    cold = torch.from_numpy(col).to(dev)
+    dev_fmt = T.partition_device(n, n, rp, col, rpd, cold)
+    t_dev = _wall(lambda: T.partition_device(n, n, rp, col, rpd, cold))
+    t_shapes = _wall(lambda: T.partition_shapes(n, n, rp, col))
+    H, W_, pm, pn = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    T.lib().tc_thumb_dims(n, n, C.byref(H), C.byref(W_), C.byref(pm), C.byref(pn))
+    img = torch.empty(H.value * W_.value, dtype=torch.uint8, device=dev)
+    T.thumbnail_gpu(n, n, rpd, cold, img)
+    img_h = img.cpu().numpy()
+    t_host = _wall(lambda: T.partition_from_image(n, n, rp, col, img=img_h))
+    host_fmt, _ = T.partition_from_image(n, n, rp, col, img=img_h)
+    same = all(getattr(dev_fmt, f) == getattr(host_fmt, f) for f in T.Format.FIELDS) and \
+        all(np.array_equal(getattr(dev_fmt, f), getattr(host_fmt, f)) for f in T.Format.ARRAYS)
+    return {"device_build_s": round(t_dev, 4), "partition_shapes_s": round(t_shapes, 4),
+            "host_build_s": round(t_host, 4), "speedup": round(t_host / t_dev, 2),
+            "device_equals_host": bool(same), "remainder_entries": int(dev_fmt.csrRow[-1])}
+
+
+def bench_loader(n, rp, col):
+    """Time to a partitioned format from disk: Matrix Market text read
+    (parallel parse) + partition_shapes, against loading the checksummed
+    format cache written from the same format."""
+    import tempfile
+    import taichi_amd as T
+    d = tempfile.mkdtemp()
+    mtx, cache = os.path.join(d, "rmat.mtx"), os.path.join(d, "rmat.tcfmt")
+    try:
+        T.write_mtx(mtx, n, n, rp, col)
+        t_read = _wall(lambda: T.read_mtx(mtx), 2)
+        rc, M, N, rp2, col2, _ = T.read_mtx(mtx)
+        assert rc == 0, rc
+        t_part = _wall(lambda: T.partition_shapes(M, N, rp2, col2), 2)
+        fmt, _ = T.partition_shapes(M, N, rp2, col2)
+        t_save = _wall(lambda: T.save_format(cache, fmt), 2)
+        t_load = _wall(lambda: T.load_format(cache), 2)
+        back = T.load_format(cache)
+        same = all(np.array_equal(getattr(back, f), getattr(fmt, f)) for f in T.Format.ARRAYS)
+        return {"mtx_bytes": os.path.getsize(mtx), "cache_bytes": os.path.getsize(cache),
+                "read_mtx_s": round(t_read, 4), "partition_s": round(t_part, 4),
+                "read_plus_partition_s": round(t_read + t_part, 4), "cache_save_s": round(t_save, 4),
+                "cache_load_s": round(t_load, 4), "cache_equals_partition": bool(same)}
+    finally:
+        for f in (mtx, cache):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
 
 
 def bench_pagerank(A, n, col, dev, iters=20):
@@ -223,7 +301,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the pagerank, batched, served and multi-RHS sections")
-    ap.add_argument("--sections", default="pagerank,batched,served,multi_rhs",
+    ap.add_argument("--sections", default="partition,loader,pagerank,batched,served,multi_rhs",
                     help="comma-separated extra sections to run")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--config", default="",
@@ -268,10 +346,7 @@ def main():
     # correctness spot check against the FP64 CSR product
     A.spmv(xd, yd)
     torch.cuda.synchronize()
-    import oracle
-    y64 = oracle.csr_spmv_f64(rp, col, x.astype(np.float64))
-    absrow = oracle.csr_absrow_f64(rp, col, x.astype(np.float64))
-    err = oracle.rel_err(yd.cpu().numpy(), y64, absrow)
+    err = _rel_err_fp64(rp, col, n, x, yd.cpu().numpy())
     log(f"max componentwise relative error vs FP64: {err:.3e}")
 
     for _ in range(args.warmup):
@@ -374,6 +449,14 @@ def main():
         "wall_s_timed": round(wall, 4),
     }
     extra = set() if args.no_extra else set(args.sections.split(","))
+    if "partition" in extra:
+        log("partition: device build vs host build")
+        out["partition"] = bench_partition(n, rp, col, dev)
+        log(f"  {out['partition']}")
+    if "loader" in extra:
+        log("loader: .mtx read + partition vs format cache")
+        out["loader"] = bench_loader(n, rp, col)
+        log(f"  {out['loader']}")
     if "pagerank" in extra:
         log("iterative driver: PageRank on the same matrix")
         out["pagerank"] = bench_pagerank(A, n, col, dev)

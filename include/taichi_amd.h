@@ -16,6 +16,7 @@
  *   tc_partition_from_image  partition_shapes   TaiChi.h:83-84 (given the image)
  *   tc_partition_shapes .... partition_shapes   TaiChi.h:83-84, TaiChi.cu:313-638
  *   tc_partition_device .... partition_dia_shape's passes, TaiChi.cu:677-879 (device)
+ *   tc_morph_device ........ getSdf's morphology   morohutils.cu:231-251 (device)
  *   tc_cal_shape_size ...... cal_shape_size     TaiChi.h:87-88, TaiChi.cu:280-311
  *   tc_cal_start_end ....... cal_start_end      TaiChi.h:103, TaiChi.cu:881-916
  *   tc_free_taichi_new ..... free_taichi_new    TaiChi.h:106, TaiChi.cu:77-118
@@ -109,6 +110,16 @@ int tc_sdf_from_image(const unsigned char *img, int H, int W, int *pts, int cap,
    (no reference counterpart: exposes the step inside getSdf for checking).
    *needed receives the int count; TC_ERR_CAPACITY when cap is too small. */
 int tc_contours(const unsigned char *img, int H, int W, int *out, int cap, int *needed);
+/* getSdf's morphology alone (morohutils.cu:231-251: 255 - mat, imfill, open
+   with the 30x1 and 1x30 lines, saturated subtraction, open with eye(5)):
+   thin[H*W] is the binary image whose contours give the shapes. */
+int tc_morph_from_image(const unsigned char *img, int H, int W, unsigned char *thin);
+/* The same morphology on the device (d_img, d_thin device arrays of H*W);
+   work: device scratch of tc_morph_work_bytes(H, W) bytes. Synchronises the
+   stream. */
+size_t tc_morph_work_bytes(int H, int W);
+int tc_morph_device(const unsigned char *d_img, int H, int W, unsigned char *d_thin,
+                    void *work, void *stream);
 void tc_cal_shape_size(int M, int N, int mF, int x1, int y1, int a1, int b1,
                        int *x, int *y, int *a, int *b, float *gain);
 void tc_cal_start_end(int M_sub, int N_sub, int width, int off, int n,

@@ -133,6 +133,9 @@ def lib():
                                           P(TaiChiNew)]),
         "tc_partition_device": (C.c_int, [C.c_int, C.c_int, C.c_int, i32, vp, vp, vp, P(TaiChiNew),
                                           vp]),
+        "tc_morph_from_image": (C.c_int, [u8, C.c_int, C.c_int, u8]),
+        "tc_morph_work_bytes": (C.c_size_t, [C.c_int, C.c_int]),
+        "tc_morph_device": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp]),
         "tc_taichi_new_alloc": (P(TaiChiNew), []),
         "tc_save_format": (C.c_int, [C.c_char_p, C.c_int, C.c_int, P(TaiChiNew)]),
         "tc_load_format": (C.c_int, [C.c_char_p, P(C.c_int), P(C.c_int), P(TaiChiNew)]),
@@ -466,6 +469,27 @@ def load_format(path):
                 L.tc_free_host(C.cast(p, C.c_void_p))
     L.tc_free_taichi_new(heap)
     return fmt
+
+
+def morph_from_image(img):
+    """getSdf's morphology on the host (tc_morph_from_image): the binary image
+    whose contours give the shapes; img is a uint8 [H, W] array."""
+    img = np.ascontiguousarray(img, np.uint8)
+    thin = np.empty_like(img)
+    check(lib().tc_morph_from_image(img.ravel(), img.shape[0], img.shape[1], thin.ravel()),
+          "morph_from_image")
+    return thin
+
+
+def morph_device(img_dev, H, W, stream=None):
+    """The same morphology on the device (tc_morph_device); img_dev is a
+    uint8 device tensor of H*W. Returns the thin image as a device tensor."""
+    import torch
+    thin = torch.empty(H * W, dtype=torch.uint8, device=img_dev.device)
+    work = torch.empty(int(lib().tc_morph_work_bytes(H, W)), dtype=torch.uint8, device=img_dev.device)
+    check(lib().tc_morph_device(img_dev.data_ptr(), H, W, thin.data_ptr(), work.data_ptr(),
+                                _torch_stream(stream)), "morph_device")
+    return thin
 
 
 def thumbnail_gpu(M, N, rowptr_dev, col_dev, img_dev, stream=0):

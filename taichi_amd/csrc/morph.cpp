@@ -248,7 +248,9 @@ namespace tc {
 
 // getSdf's image pipeline (morohutils.cu:231-282): returns x1,y1,x2,y2 per
 // kept contour (contour[0] and contour[size/2], size >= 80).
-std::vector<int> sdf_from_image(const uint8_t *mat, int H, int W)
+// getSdf's morphology (morohutils.cu:231-251): the binary image whose
+// contours give the shapes.
+void morph_thin(const uint8_t *mat, int H, int W, uint8_t *thin_out)
 {
     const size_t n = (size_t)H * W;
     Img rev(n), fill(n), o1(n), o2(n), del(n), thin(n);
@@ -258,7 +260,15 @@ std::vector<int> sdf_from_image(const uint8_t *mat, int H, int W)
     open_line30(o1, o2, H, W, false);
     for (size_t i = 0; i < n; i++) del[i] = rev[i] > o2[i] ? (uint8_t)(rev[i] - o2[i]) : 0;
     open_eye5(del, thin, H, W);
-    std::vector<Contour> cs = find_contours_ccomp(thin, H, W);
+    memcpy(thin_out, thin.data(), n);
+}
+
+// x1,y1,x2,y2 of each contour of at least 80 points (morohutils.cu:138-151,
+// 257-270): its first point and the one halfway along.
+std::vector<int> contour_points(const uint8_t *thin, int H, int W)
+{
+    Img bin(thin, thin + (size_t)H * W);
+    std::vector<Contour> cs = find_contours_ccomp(bin, H, W);
     std::vector<int> pts;
     for (const Contour &c : cs) {
         int len = (int)c.xy.size() / 2;
@@ -269,6 +279,13 @@ std::vector<int> sdf_from_image(const uint8_t *mat, int H, int W)
         pts.push_back(c.xy[2 * (len / 2) + 1]);
     }
     return pts;
+}
+
+std::vector<int> sdf_from_image(const uint8_t *mat, int H, int W)
+{
+    Img thin((size_t)H * W);
+    morph_thin(mat, H, W, thin.data());
+    return contour_points(thin.data(), H, W);
 }
 
 // Contours alone, for tests: flattened as [n, (len, hole, x0,y0,...)...].
@@ -296,6 +313,13 @@ extern "C" int tc_sdf_from_image(const unsigned char *img, int H, int W, int *pt
     *num_shapes = k;
     if (k > cap) return tc::fail(TC_ERR_CAPACITY, "tc_sdf_from_image: %d shapes > cap %d", k, cap);
     if (k) memcpy(pts, v.data(), v.size() * sizeof(int));
+    return TC_OK;
+}
+
+extern "C" int tc_morph_from_image(const unsigned char *img, int H, int W, unsigned char *thin)
+{
+    if (!img || !thin || H <= 0 || W <= 0) return tc::fail(TC_ERR_INVALID, "tc_morph_from_image: bad argument");
+    tc::morph_thin(img, H, W, thin);
     return TC_OK;
 }
 

@@ -54,13 +54,19 @@ int tc::width_tier(int sd)
 // corners scaled by cal_shape_size, too-small ones dropped.
 int tc::candidate_squares(int M, int N, const unsigned char *img, std::vector<Square> &out)
 {
-    out.clear();
     int H, W, pM, pN;
     std::vector<int> pts;
     if (tc_thumb_dims(M, N, &H, &W, &pM, &pN) == 0) {
         if (!img) return tc::fail(TC_ERR_INVALID, "partition: thumbnail required for M > 512");
         pts = tc::sdf_from_image(img, H, W);
     }
+    candidate_squares_from_pts(M, N, pts, out);
+    return TC_OK;
+}
+
+void tc::candidate_squares_from_pts(int M, int N, const std::vector<int> &pts, std::vector<Square> &out)
+{
+    out.clear();
     const int num_shapes = (int)pts.size() / 4;
     for (int i = 0; i < num_shapes; i++) {
         int x1 = pts[4 * i], y1 = pts[4 * i + 1];
@@ -71,7 +77,6 @@ int tc::candidate_squares(int M, int N, const unsigned char *img, std::vector<Sq
         if (a < kMinShape || b < kMinShape) continue;
         out.push_back({x, y, b});
     }
-    return TC_OK;
 }
 
 // The density test of partition_dia_shape (TaiChi.cu:736-760) from the

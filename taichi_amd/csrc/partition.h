@@ -12,6 +12,9 @@ namespace tc {
 
 // getSdf's image pipeline: x1,y1,x2,y2 per kept contour
 std::vector<int> sdf_from_image(const uint8_t *mat, int H, int W);
+// its two halves: the morphology (binary image out) and the contour points
+void morph_thin(const uint8_t *mat, int H, int W, uint8_t *thin);
+std::vector<int> contour_points(const uint8_t *thin, int H, int W);
 // contours alone: [n, (len, hole, x0, y0, ...)...]
 std::vector<int> contours_flat(const uint8_t *img, int H, int W);
 
@@ -32,6 +35,7 @@ struct Square {
 };
 int width_tier(int side);
 int candidate_squares(int M, int N, const unsigned char *img, std::vector<Square> &out);
+void candidate_squares_from_pts(int M, int N, const std::vector<int> &pts, std::vector<Square> &out);
 void decide_dense(int side, int w, const long long *cnt, DiaShape &sh, std::vector<int> &dense_of);
 void shape_arrays(const std::vector<DiaShape> &kept, int nnz, TaiChi_new *out);
 

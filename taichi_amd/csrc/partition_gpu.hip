@@ -1,8 +1,9 @@
-// Partition and format build on the device. The O(nnz) passes of
+// Partition and format build on the device (thumbnail and its morphology
+// included: thumb.hip, morph_gpu.hip). The O(nnz) passes of
 // partition_dia_shape run as kernels: the per-diagonal histogram
 // (TaiChi.cu:723-734), the claim of dense-diagonal entries (:850-871), and
 // the zero rows (:809-827). So does the merge into the encoded CSR
-// (:480-612). The decisions (contours of the thumbnail, cal_shape_size, the
+// (:480-612). The decisions (contours of the morphed thumbnail, cal_shape_size, the
 // FP32 density test, the shape arrays) run on the host through the same
 // functions as the host build (partition.cpp), so the TaiChi_new is the same.
 //
@@ -172,11 +173,10 @@ __global__ void narrow_rows(int n, const long long *__restrict__ a, int *__restr
 // The passes above for the squares of one thumbnail; fills out like
 // tc_partition_from_image with a NULL val.
 int partition_on_device(int M, int N, int nnz, int *rowptr, int *col, const int *d_rowptr, const int *d_col,
-                        const unsigned char *img, TaiChi_new *out, hipStream_t s)
+                        const std::vector<int> &pts, TaiChi_new *out, hipStream_t s)
 {
     std::vector<tc::Square> cand;
-    int rc = tc::candidate_squares(M, N, img, cand);
-    if (rc != TC_OK) return rc;
+    tc::candidate_squares_from_pts(M, N, pts, cand);
     int max_nd = 1;
     for (const tc::Square &q : cand) max_nd = std::max(max_nd, 2 * tc::width_tier(q.side) - 1);
 
@@ -333,15 +333,13 @@ extern "C" int tc_partition_device(int M, int N, int nnz, int *rowptr, int *col,
                         rowptr[M], nnz);
     hipStream_t s = (hipStream_t)stream;
     int H, W, pM, pN;
-    std::vector<unsigned char> img;
+    std::vector<int> pts;  // contour points; none when M <= 512 (no thumbnail)
     if (tc_thumb_dims(M, N, &H, &W, &pM, &pN) == 0) {
         DevMem dimg;
-        img.resize((size_t)H * W);
-        DCHECK(dimg.alloc(img.size()));
+        DCHECK(dimg.alloc((size_t)H * W));
         int rc = tc_thumbnail(M, N, d_rowptr, d_col, dimg.as<unsigned char>(), s);
+        if (rc == TC_OK) rc = tc::sdf_on_device(dimg.as<unsigned char>(), H, W, pts, s);
         if (rc != TC_OK) return rc;
-        DCHECK(hipMemcpyAsync(img.data(), dimg.p, img.size(), hipMemcpyDeviceToHost, s));
-        DCHECK(hipStreamSynchronize(s));
     }
-    return partition_on_device(M, N, nnz, rowptr, col, d_rowptr, d_col, img.empty() ? nullptr : img.data(), out, s);
+    return partition_on_device(M, N, nnz, rowptr, col, d_rowptr, d_col, pts, out, s);
 }

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "spmv_tiles.h"
@@ -28,6 +29,12 @@ struct tc_matrix {
     std::mutex mu;
     std::map<std::pair<int, int>, tc_config> cfg;
 };
+
+namespace tc {
+// getSdf with the morphology on the device (morph_gpu.hip): the points
+// tc_sdf_from_image gives for the same thumbnail, d_img in device memory
+int sdf_on_device(const unsigned char *d_img, int H, int W, std::vector<int> &pts, hipStream_t s);
+}  // namespace tc
 
 // a captured launch sequence (tc_graph_capture, tc_pagerank_capture)
 struct tc_graph {

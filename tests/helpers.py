@@ -49,3 +49,49 @@ def run_spmv(fmt, x, dev, k=None, dtype=None):
     A.spmv(xd, yd)
     torch.cuda.synchronize()
     return yd.cpu().numpy(), A
+
+
+def morph_images():
+    """Thumbnails for the morphology checks: (name, uint8 [H, W], 0 = hit)."""
+    out = []
+    for seed in range(6):
+        rng = np.random.default_rng(300 + seed)
+        H, W = 512, int(rng.integers(300, 513))
+        a = np.full((H, W), 255, np.uint8)
+        for _ in range(int(rng.integers(1, 6))):        # diagonal segments
+            r0, c0, ln = rng.integers(0, 400), rng.integers(0, W - 50), rng.integers(60, 300)
+            t = np.arange(int(ln))
+            keep = (r0 + t < H) & (c0 + t < W)
+            a[r0 + t[keep], c0 + t[keep]] = 0
+        a[rng.random((H, W)) < rng.uniform(0.0, 0.05)] = 0
+        if seed % 2:
+            a[100:160, 200:260] = 0                       # a dense block
+        out.append((f"random{seed}", a))
+    a = np.full((512, 512), 255, np.uint8)                # nested closed rings: holes in holes
+    for k in range(0, 240, 12):
+        a[k, k:512 - k] = 0
+        a[511 - k, k:512 - k] = 0
+        a[k:512 - k, k] = 0
+        a[k:512 - k, 511 - k] = 0
+    out.append(("rings", a))
+    a = np.full((512, 512), 255, np.uint8)                # rings with alternating gaps: a maze
+    for i, k in enumerate(range(2, 250, 4)):
+        a[k, k:512 - k] = 0
+        a[511 - k, k:512 - k] = 0
+        a[k:512 - k, k] = 0
+        a[k:512 - k, 511 - k] = 0
+        g = k + 2 if i % 2 else 509 - k
+        a[k, g] = 255                                     # gap on the top side, alternating ends
+    out.append(("maze", a))
+    yy, xx = np.mgrid[0:512, 0:512]
+    out.append(("checker", (((yy + xx) % 2) * 255).astype(np.uint8)))
+    out.append(("all_hit", np.zeros((512, 512), np.uint8)))
+    out.append(("empty", np.full((512, 512), 255, np.uint8)))
+    rng = np.random.default_rng(7)
+    for W in (300, 700):
+        a = np.full((512, W), 255, np.uint8)
+        for i in range(min(512, W)):
+            a[i, i] = 0
+        a[rng.random((512, W)) < 0.02] = 0
+        out.append((f"wide{W}", a))
+    return out

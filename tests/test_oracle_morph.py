@@ -9,6 +9,8 @@ import pytest
 import oracle as O
 import taichi_amd as T
 
+from helpers import morph_images
+
 H30 = (np.arange(30) - 15, np.zeros(30, int))
 
 
@@ -194,3 +196,15 @@ def test_sdf_of_single_diagonal_thumbnail():
         a[i, i] = 0
     for sdf in (O.sdf_from_image(a), T.sdf_from_image(a)):
         assert sdf.tolist() == [[0, 0, 511, 511]]
+
+
+@pytest.mark.parametrize("case", morph_images(), ids=lambda c: c[0])
+def test_product_morph_matches_oracle_steps(case):
+    """tc_morph_from_image equals the oracle's steps composed as getSdf
+    composes them (morohutils.cu:231-251), pixel for pixel."""
+    _, a = case
+    rev = (255 - a.astype(np.int32)).astype(np.uint8)
+    o2 = O.open_(O.open_(O.imfill(rev), 0), 1)
+    dele = np.where(rev > o2, rev.astype(np.int32) - o2, 0).astype(np.uint8)
+    want = O.open_(dele, 2)
+    assert np.array_equal(T.morph_from_image(a), want)
