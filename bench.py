@@ -27,20 +27,43 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(rp, col, x, seconds=10.0):
-    """Reference CPU CSR SpMV (main.cu:61-69), single core, from the oracle."""
-    import oracle
-    nnz = len(col)
+def _cpu_median(fn, seconds, min_runs=5):
+    """median wall time of fn() over runs filling about `seconds`"""
     times = []
     t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 5:
+    while time.perf_counter() < t_end or len(times) < min_runs:
         t0 = time.perf_counter()
-        oracle.csr_spmv_f32(rp, col, x)
+        fn()
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return {"value": nnz / med / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
-            "sample": f"full R-MAT 2^22 matrix ({nnz} nnz), {len(times)} sequential FP32 "
-                      f"CSR SpMVs, median {med * 1e3:.2f} ms"}
+    return float(np.median(times)), len(times)
+
+
+def cpu_baseline(rp, col, x, seconds=10.0):
+    """Reference CPU CSR SpMV (main.cu:61-69) from the oracle: single core on
+    the headline matrix (the reported value), the same sums with rows split
+    over the host cores this process may use (at most 16), and both forms on
+    the 16,384-row five-diagonal matrix and the 2048^2 stencil at k = 1."""
+    import oracle
+    from taichi_amd import workloads as W
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    nnz = len(col)
+    med, runs = _cpu_median(lambda: oracle.csr_spmv_f32(rp, col, x), seconds)
+    med_mt, runs_mt = _cpu_median(lambda: oracle.csr_spmv_f32_mt(rp, col, x, threads), seconds / 4)
+    out = {"value": nnz / med / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
+           "sample": f"full R-MAT 2^22 matrix ({nnz} nnz), {runs} sequential FP32 "
+                     f"CSR SpMVs, median {med * 1e3:.2f} ms",
+           "multi_core": {"value": nnz / med_mt / 1e9, "cores": threads,
+                          "sample": f"{runs_mt} SpMVs, rows split statically, median {med_mt * 1e3:.2f} ms"},
+           "configs": {}}
+    for name, (M, N, rp2, col2) in (("diag5_random_16384", W.diag5_random()),
+                                    ("stencil5_2048", W.stencil5())):
+        x2 = np.random.default_rng(5).random(N).astype(np.float32)
+        m1, _ = _cpu_median(lambda: oracle.csr_spmv_f32(rp2, col2, x2), seconds / 8)
+        mm, _ = _cpu_median(lambda: oracle.csr_spmv_f32_mt(rp2, col2, x2, threads), seconds / 16)
+        out["configs"][name] = {"nnz": len(col2), "single_core_gnnz_per_s": round(len(col2) / m1 / 1e9, 4),
+                                "multi_core_gnnz_per_s": round(len(col2) / mm / 1e9, 4),
+                                "single_core_ms": round(m1 * 1e3, 4), "multi_core_ms": round(mm * 1e3, 4)}
+    return out
 
 
 def read_traffic(name):
@@ -266,6 +289,30 @@ def bench_loader(n, rp, col):
         os.rmdir(d)
 
 
+def bench_small(dev, steps=50):
+    """The 16,384-row five-diagonal matrix with 16,384 uniform draws (seed
+    0), x uniform integers 0..7: y must equal the CPU CSR sums exactly; the
+    whole working set sits in L2, so this measures latency."""
+    import scipy.sparse as sp
+    import torch
+    import taichi_amd as T
+    from taichi_amd import workloads as W
+    M, N, rp, col = W.diag5_random()
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    A = T.Matrix(fmt, nnz=len(col))
+    x = T.gen_vector(N, 1, 3)  # integers 0..7 as FP32
+    xd = torch.from_numpy(x).to(dev)
+    yd = torch.empty(M, device=dev)
+    stream = torch.cuda.current_stream()
+    g = T.Graph(A, xd, yd, iters=steps, stream=stream.cuda_stream)
+    ms = _time_events(lambda: g.launch(stream=stream.cuda_stream), 5, stream) / steps
+    g.close()
+    ref = (sp.csr_matrix((np.ones(len(col)), col, rp), shape=(M, N)) @ x.astype(np.float64)).astype(np.float32)
+    return {"workload": "n=16384, ones at offsets -2..2 plus 16384 uniform draws (seed 0)", "nnz": len(col),
+            "us_per_spmv": round(ms * 1e3, 3), "gnnz_per_s": round(len(col) / ms / 1e6, 2),
+            "bit_exact_integer_x": bool(np.array_equal(yd.cpu().numpy(), ref))}
+
+
 def bench_pagerank(A, n, col, dev, iters=20):
     """PageRank power iteration (alpha 0.85) on the resident matrix: `iters`
     iterations enqueued eagerly and replayed from one captured hipGraph; each
@@ -301,7 +348,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the pagerank, batched, served and multi-RHS sections")
-    ap.add_argument("--sections", default="partition,loader,pagerank,batched,served,multi_rhs",
+    ap.add_argument("--sections", default="small,partition,loader,pagerank,batched,served,multi_rhs",
                     help="comma-separated extra sections to run")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--config", default="",
@@ -448,7 +495,14 @@ def main():
         "max_rel_err_vs_fp64": err,
         "wall_s_timed": round(wall, 4),
     }
+    props = torch.cuda.get_device_properties(0)
+    out["device"] = {"name": props.name, "arch": getattr(props, "gcnArchName", ""),
+                     "compute_units": props.multi_processor_count, "hbm_bytes": props.total_memory}
     extra = set() if args.no_extra else set(args.sections.split(","))
+    if "small" in extra:
+        log("small: 16,384-row five-diagonal matrix")
+        out["small"] = bench_small(dev)
+        log(f"  {out['small']}")
     if "partition" in extra:
         log("partition: device build vs host build")
         out["partition"] = bench_partition(n, rp, col, dev)

@@ -63,6 +63,7 @@ def lib():
                                            C.c_void_p, C.c_void_p, P(OrcTaiChiNew)]
         L.orc_free_taichi_new_arrays.argtypes = [P(OrcTaiChiNew)]
         L.orc_csr_spmv_f32.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p]
+        L.orc_csr_spmv_f32_mt.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, C.c_int]
         L.orc_csr_spmv_f64.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p]
         L.orc_csr_absrow_f64.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p]
         L.orc_csr_spmm_f64.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, C.c_int]
@@ -264,6 +265,16 @@ def csr_spmv_f32(rowptr, col, x):
     lib().orc_csr_spmv_f32(M, np.ascontiguousarray(rowptr, np.int32),
                            np.ascontiguousarray(col, np.int32),
                            np.ascontiguousarray(x, np.float32), y)
+    return y
+
+
+def csr_spmv_f32_mt(rowptr, col, x, threads):
+    """csr_spmv_f32 with rows split over `threads` OpenMP threads (same y)."""
+    M = len(rowptr) - 1
+    y = np.empty(M, np.float32)
+    lib().orc_csr_spmv_f32_mt(M, np.ascontiguousarray(rowptr, np.int32),
+                              np.ascontiguousarray(col, np.int32),
+                              np.ascontiguousarray(x, np.float32), y, int(threads))
     return y
 
 

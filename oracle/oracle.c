@@ -793,6 +793,19 @@ void orc_csr_spmv_f32(int M, const int *rowptr, const int *col, const float *x, 
     }
 }
 
+/* The same sums with rows split statically over `threads` OpenMP threads:
+   each row is still summed sequentially, so y equals orc_csr_spmv_f32's.
+   The CPU baseline across host cores. */
+void orc_csr_spmv_f32_mt(int M, const int *rowptr, const int *col, const float *x, float *y, int threads)
+{
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int i = 0; i < M; i++) {
+        float s = 0.0f;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++) s += x[col[j]];
+        y[i] = s;
+    }
+}
+
 void orc_csr_spmv_f64(int M, const int *rowptr, const int *col, const double *x, double *y)
 {
     for (int i = 0; i < M; i++) {

@@ -106,6 +106,7 @@ void orc_free_taichi_new_arrays(orc_taichi_new *t);
 /* ---- a13: CPU CSR SpMV (main.cu:61-69), FP64 oracle, format interpreter ---- */
 void orc_csr_spmv_f32(int M, const int *rowptr, const int *col,
                       const float *x, float *y);
+void orc_csr_spmv_f32_mt(int M, const int *rowptr, const int *col, const float *x, float *y, int threads);
 void orc_csr_spmv_f64(int M, const int *rowptr, const int *col,
                       const double *x, double *y);
 /* per-row sum of |x[col]| over the CSR (componentwise error bound) */

@@ -111,6 +111,9 @@ def test_csr_spmv_f32_is_sequential_float32():
     for i in range(M):
         col_r[rp[i]:rp[i + 1]] = col[rp[i]:rp[i + 1]][::-1]
     assert not np.array_equal(O.csr_spmv_f32(rp, col_r, x), want)
+    # the multi-threaded baseline splits rows, not sums: the same y
+    for threads in (1, 3, 8):
+        assert np.array_equal(O.csr_spmv_f32_mt(rp, col, x, threads), want)
 
 
 # ---- golden partition derived by hand (tests/golden/make_golden.py) ----
