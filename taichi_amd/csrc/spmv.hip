@@ -47,6 +47,21 @@ tc_config default_config(int K, long long remainder, int M)
 
 tc_config default_config(const tc_matrix *A, int K) { return default_config(K, A->st.remainder, A->v.M); }
 
+size_t max_lds_bytes()
+{
+    static std::mutex mu;
+    static std::map<int, size_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    hipDeviceProp_t prop;
+    const size_t v = hipGetDeviceProperties(&prop, dev) == hipSuccess ? (size_t)prop.sharedMemPerBlock : 65536;
+    cache[dev] = v;
+    return v;
+}
+
 bool rows_per_tile_ok(int K, int r, int flags)
 {
     if (K == 1) return r == 256 || r == 512 || r == 1024 || r == 2048 || r == 4096;

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restr
     extern __shared__ float smem[];
     __shared__ int s_ub[kBatchLdsDesc];
     float *s_val = smem;
-    float *s_lsum = s_val + capv;
+    double *s_lsum = (double *)(s_val + lsum_at(capv));
     int *s_start = (int *)(s_lsum + kWG * kBatchP);
     batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
         run_tile<K, XT, kBatchP>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t * TRW, s_val, capv, s_start, s_lsum);
@@ -160,10 +160,11 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
     const int grid = units;  // one workgroup per tile: the dispatcher balances chunks of any size
     if (kern == 0) {
         constexpr int TRW = kWG * kBatchRPT;
-        const size_t lds = ((size_t)kBatchCapv + 2 * TRW) * 4;
+        const size_t lds = ((size_t)lsum_at(kBatchCapv) + 3 * TRW) * 4;  // s_val | long sums (doubles) | window
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     } else if (kern == 1) {
-        const size_t lds = (size_t)(wide_rows(K, kBatchRI) + kWideHalo) * K * 4;
+        const size_t lds = std::max((size_t)(wide_rows(K, kBatchRI) + kWideHalo) * K * 4,
+                                    (size_t)wide_rows(K, kBatchRI) * K * 8);
         if (K == 2) batch_wide_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
         else if (K == 4) batch_wide_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
         else batch_wide_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);

@@ -26,7 +26,8 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
 // (texture addresser busy most of the kernel), not by bytes. Same canonical
 // per-row order as every other kernel. Requires X and Y aligned to Q
 // elements (checked on the host).
-// LDS: s_val[capv] (also the long-row sums afterwards) | s_win[TRW] (DIAG).
+// LDS: s_val[capv] (also the long-row sums afterwards: TRW doubles, so capv
+// >= 2*TRW when the matrix has long rows) | s_win[TRW] (DIAG).
 template <int OW, int Q>
 __device__ __forceinline__ void load_ends(const void *roff, int r0, int (&e)[Q])
 {
@@ -97,16 +98,16 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
     int4 cv = make_int4(1, 1, 1, 1);
     if (4 * tid < min(EA, capv)) cv = *(const int4 *)(A.cols + lb + 4 * tid);
 
-    float acc[Q], cmp[Q];
+    double acc[Q];
 #pragma unroll
-    for (int i = 0; i < Q; i++) acc[i] = 0.0f, cmp[i] = 0.0f;
+    for (int i = 0; i < Q; i++) acc[i] = 0.0;
     if constexpr (DIAG) {
         const ShapeDev &sh = A.sh0;
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int r = r0 + i, c = w0 + Q * tid + i;
             if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs))
-                cadd(acc[i], cmp[i], xw[i]);
+                accum(acc[i], xw[i]);
         }
         if constexpr (Q == 4) *(float4 *)(s_win + Q * tid) = make_float4(xw[0], xw[1], xw[2], xw[3]);
         else *(float2 *)(s_win + Q * tid) = make_float2(xw[0], xw[1]);
@@ -149,10 +150,9 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int lo = max(rs[i], e0), hi = min(re[i], e0 + ne);
-            float a = acc[i], e = cmp[i];
-            for (int k = lo; k < hi; k++) cadd(a, e, s_val[k - e0]);
+            double a = acc[i];
+            for (int k = lo; k < hi; k++) accum(a, s_val[k - e0]);
             acc[i] = a;
-            cmp[i] = e;
         }
         __syncthreads();
     }
@@ -162,17 +162,17 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
         bool pv[Q];
 #pragma unroll
         for (int i = 0; i < Q; i++) prow[i] = Q * tid + i, lq[i] = prow[i], pv[i] = r0 + i < A.M;
-        long_phase<1, XT, Q>(A, X, 1, u0, min(TRW, A.M - u0), lq, pv, acc, cmp, s_val, TRW);
+        long_phase<1, XT, Q>(A, X, 1, u0, min(TRW, A.M - u0), lq, pv, acc, (double *)s_val, TRW);
     }
 
     if (full) {
         if constexpr (Q == 4)
-            *(float4 *)(Y + r0) = make_float4(acc[0] + cmp[0], acc[1] + cmp[1], acc[2] + cmp[2], acc[3] + cmp[3]);
-        else *(float2 *)(Y + r0) = make_float2(acc[0] + cmp[0], acc[1] + cmp[1]);
+            *(float4 *)(Y + r0) = make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+        else *(float2 *)(Y + r0) = make_float2((float)acc[0], (float)acc[1]);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; i++)
-            if (r0 + i < A.M) Y[r0 + i] = acc[i] + cmp[i];
+            if (r0 + i < A.M) Y[r0 + i] = (float)acc[i];
     }
 }
 
@@ -205,7 +205,9 @@ int launch_rows_t(const MatView &A, const void *X, float *Y, int capv, int grid,
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
     const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1;
-    const size_t lds = ((size_t)capv + (A.L > 0 ? TRW : 0) + (fast ? TRW : 0)) * 4;
+    const size_t lds = ((size_t)lsum_at(capv) + (A.L > 0 ? 2 * TRW : 0) + (fast ? TRW : 0)) * 4;
+    if (lds > max_lds_bytes())
+        return tc::fail(TC_ERR_UNSUPPORTED, "rows tiles of %d with %d LDS values need %zu B of LDS", TRW, capv, lds);
     const bool lng = A.L > 0;
     // specialised on offset width, the one-shape-one-offset case and long
     // rows, so the common case carries no registers for the general paths
@@ -277,6 +279,7 @@ bool quad_applies(const MatView &A, const void *X, const float *Y, int esize, co
     if (!(A.S == 0 || (A.S == 1 && A.sh0.oe - A.sh0.ob == 1))) return false;
     const int Q = c.rows_per_tile == 512 ? 2 : c.rows_per_tile == 1024 ? 4 : 0;
     if (Q == 0 || capv % 4 != 0 || capv < kWG * Q) return false;
+    if (A.L > 0 && capv < 2 * kWG * Q) return false;  // the long-row sums (doubles) reuse s_val
     return ((uintptr_t)X % ((uintptr_t)Q * esize)) == 0 && ((uintptr_t)Y % ((uintptr_t)Q * 4)) == 0;
 }
 

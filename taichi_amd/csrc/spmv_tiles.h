@@ -20,14 +20,14 @@
 //
 // A workgroup (4 wavefronts) computes a tile of rows. Every kernel adds a
 // (row, column) pair's terms in one canonical order fixed by the format:
-//   1. diagonal terms, shape by shape; within a shape the offsets in blocks of
-//      kDiagBlock by offset index, each block summed with plain adds and the
-//      block sum added compensated; the tile's x window is staged in LDS when
-//      a shape has two or more offsets, so no column index is read for it;
+//   1. diagonal terms, shape by shape, offsets in stored order; the tile's x
+//      window is staged in LDS when a shape has two or more offsets, so no
+//      column index is read for it;
 //   2. remainder entries in stored order (streamed with coalesced loads, x
 //      gathered and signed, staged in LDS, summed by one lane per row);
 //   3. long rows: one wavefront each, 64-lane strided partial sums and a
 //      fixed DPP/ds_swizzle butterfly.
+// Every term is added in FP64 (accum) and y is rounded to FP32 once.
 // The order never depends on the tile size, the grid, the column chunking or
 // the batch a request runs in, so a request gives bit-identical y alone, in a
 // batch, or with another configuration.
@@ -44,13 +44,9 @@ namespace tcd {
 constexpr int kWG = 256;
 constexpr int kWaves = kWG / 64;
 constexpr int kStageU = 8;  // loads in flight per lane when staging streams
-// Diagonal terms of a shape are taken in blocks of kDiagBlock offsets (by
-// offset index from the shape's first): a block is summed with plain FP32
-// adds in offset order and its sum enters the compensated accumulator once.
-// A block of 4 adds at most 3 ulps-relative error to its terms; the
-// compensated part keeps the cancellation against the remainder's zero
-// entries exact.
-constexpr int kDiagBlock = 4;
+
+// Offset (in floats) of a double array placed after capv floats of LDS
+__host__ __device__ constexpr int lsum_at(int capv) { return (capv + 1) & ~1; }
 
 struct alignas(16) ShapeDev {
     int xs, xe, ys, ob, oe, omin, omax, pad;
@@ -127,32 +123,47 @@ __device__ __forceinline__ int row_start(const MatView &A, int i)
     return b;
 }
 
-// Sum over the 64 lanes of a wavefront in a fixed order: quad swaps and
-// row mirrors by DPP, the 16-lane swap by ds_swizzle, halves by readlane.
-__device__ __forceinline__ float wave_sum(float v)
+// Accumulation: every term (an FP32 value) is converted to FP64 and added
+// to a double in the canonical order; y is that double rounded to FP32 once.
+// A sum of n FP32 terms in FP64 is within n * 2^-53 * sum|terms| of the
+// exact sum, so the x values a dense diagonal adds at its zero cells and the
+// remainder subtracts again cancel to far below one FP32 ulp of the row's
+// true terms, and y is the FP32 rounding of a nearly exact sum. Integer-valued
+// terms stay exact. On gfx950 a convert plus a v_add_f64 costs less than a
+// compensated FP32 TwoSum (measured 16.7 vs 9.3 G terms/s).
+__device__ __forceinline__ void accum(double &s, float x) { s += (double)x; }
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
 {
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
-    v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));                      // lane ^ 16
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)) +
-           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Compensated accumulation: s + c carries the running sum with the exact
-// rounding error of every add (Knuth's TwoSum, branch-free) kept in c, the
-// same error term Neumaier's magnitude-ordered form computes. Dense diagonals
-// add x over whole segments and the remainder subtracts it again at their
-// zero cells, so plain FP32 sums lose accuracy to cancellation; the
-// compensated sum keeps each y within a few ulps of the exact sum.
-// Integer-valued inputs stay exact (c stays 0). Needs -ffp-contract=off.
-__device__ __forceinline__ void cadd(float &s, float &c, float x)
+// Sum over the 64 lanes of a wavefront in a fixed order, in FP64: quad swaps
+// and row mirrors by DPP, the 16-lane swap by ds_swizzle, halves by readlane;
+// each 64-bit value moves as two 32-bit halves. Every lane ends with the same
+// value (each pairwise add sees the same two operands).
+__device__ __forceinline__ double wave_sum_d(double v)
 {
-    const float t = s + x;
-    const float z = t - s;
-    c += (s - (t - z)) + (x - z);
-    s = t;
+    v += dpp_d<0xB1>(v);   // quad_perm 1,0,3,2
+    v += dpp_d<0x4E>(v);   // quad_perm 2,3,0,1
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    v += dpp_d<0x140>(v);  // row_mirror
+    {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_ds_swizzle((int)b, 0x401F);  // lane ^ 16
+        const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+        v += __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+    const long long b = __double_as_longlong(v);
+    const long long l0 = ((long long)__builtin_amdgcn_readlane((int)(b >> 32), 0) << 32) |
+                         (unsigned)__builtin_amdgcn_readlane((int)b, 0);
+    const long long l32 = ((long long)__builtin_amdgcn_readlane((int)(b >> 32), 32) << 32) |
+                          (unsigned)__builtin_amdgcn_readlane((int)b, 32);
+    return __longlong_as_double(l0) + __longlong_as_double(l32);
 }
 
 // Bijection of block ids so that each XCD (blocks are dispatched round-robin
@@ -217,13 +228,13 @@ __device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne
 }
 
 // Diagonal terms of one shape for the tile's pairs, in canonical order
-// (offsets as stored), added to (acc, cmp). The x window of a shape with two
+// (offsets as stored), added to acc. The x window of a shape with two
 // or more offsets is staged in LDS once per tile (s_val, left free after).
 template <int K, typename XT, int P>
 __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
                                                const ShapeDev &sh, int ra, int rb, const int (&prow)[P],
-                                               const int (&pcol)[P], const bool (&pv)[P], float (&acc)[P],
-                                               float (&cmp)[P], float *s_val, int capv)
+                                               const int (&pcol)[P], const bool (&pv)[P], double (&acc)[P],
+                                               float *s_val, int capv)
 {
     const int tid = threadIdx.x;
     const int shift = sh.ys - sh.xs;
@@ -250,9 +261,8 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
         __syncthreads();
     }
     // Offsets outer, pairs inner: each offset is one scalar load and the P
-    // reads behind it are independent. Each pair sums the terms of a block
-    // of kDiagBlock offsets in offset order with plain adds and adds the
-    // block sum to (acc, cmp) compensated (the canonical order).
+    // reads behind it are independent; each pair adds its terms in offset
+    // order (the canonical order).
     bool act[P];
     int base[P];
     bool inner = true;  // all of this lane's pairs active, all columns inside the square
@@ -268,54 +278,30 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
 #pragma unroll
         for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
         if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
-            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
-                const int n1 = min(n0 + kDiagBlock, sh.oe);
-                float bs[P];
+            for (int n = sh.ob; n < sh.oe; n++) {
+                const int o = cload(A.offs, n) * K;
 #pragma unroll
-                for (int p = 0; p < P; p++) bs[p] = 0.0f;
-                for (int n = n0; n < n1; n++) {
-                    const int o = cload(A.offs, n) * K;
-#pragma unroll
-                    for (int p = 0; p < P; p++) bs[p] += s_val[li[p] + o];
-                }
-#pragma unroll
-                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+                for (int p = 0; p < P; p++) accum(acc[p], s_val[li[p] + o]);
             }
         } else {
-            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
-                const int n1 = min(n0 + kDiagBlock, sh.oe);
-                float bs[P];
-#pragma unroll
-                for (int p = 0; p < P; p++) bs[p] = 0.0f;
-                for (int n = n0; n < n1; n++) {
-                    const int o = cload(A.offs, n);
-#pragma unroll
-                    for (int p = 0; p < P; p++) {
-                        const int c = base[p] + o;
-                        if (act[p] && c >= clo && c < chi) bs[p] += s_val[li[p] + o * K];
-                    }
-                }
-#pragma unroll
-                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
-            }
-        }
-        __syncthreads();
-    } else {
-        for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
-            const int n1 = min(n0 + kDiagBlock, sh.oe);
-            float bs[P];
-#pragma unroll
-            for (int p = 0; p < P; p++) bs[p] = 0.0f;
-            for (int n = n0; n < n1; n++) {
+            for (int n = sh.ob; n < sh.oe; n++) {
                 const int o = cload(A.offs, n);
 #pragma unroll
                 for (int p = 0; p < P; p++) {
                     const int c = base[p] + o;
-                    if (act[p] && c >= clo && c < chi) bs[p] += load_x<XT>(X, (size_t)c * ldx + pcol[p]);
+                    if (act[p] && c >= clo && c < chi) accum(acc[p], s_val[li[p] + o * K]);
                 }
             }
+        }
+        __syncthreads();
+    } else {
+        for (int n = sh.ob; n < sh.oe; n++) {
+            const int o = cload(A.offs, n);
 #pragma unroll
-            for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+            for (int p = 0; p < P; p++) {
+                const int c = base[p] + o;
+                if (act[p] && c >= clo && c < chi) accum(acc[p], load_x<XT>(X, (size_t)c * ldx + pcol[p]));
+            }
         }
     }
 }
@@ -323,29 +309,30 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
 template <int K, typename XT, int P>
 __device__ __forceinline__ void diag_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
                                            const int (&prow)[P], const int (&pcol)[P], const bool (&pv)[P],
-                                           float (&acc)[P], float (&cmp)[P], float *s_val, int capv)
+                                           double (&acc)[P], float *s_val, int capv)
 {
     for (int s = 0; s < A.S; s++) {
         const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
         const int ra = max(u0, sh.xs), rb = min(u0 + rows, sh.xe);
         if (ra >= rb) continue;
-        diag_one_shape<K, XT, P>(A, X, ldx, u0, s, sh, ra, rb, prow, pcol, pv, acc, cmp, s_val, capv);
+        diag_one_shape<K, XT, P>(A, X, ldx, u0, s, sh, ra, rb, prow, pcol, pv, acc, s_val, capv);
     }
 }
 
-// Long rows of the tile: one wavefront each, partial sums strided over the
-// 64 lanes and reduced by wave_sum; results land in s_lsum[local_row*K + c]
-// and are added to acc. Uniform control flow; leaves s_lsum free.
+// Long rows of the tile: one wavefront each, FP64 partial sums strided over
+// the 64 lanes and reduced by wave_sum_d; results land in s_lsum[local_row*K
+// + c] (doubles, 8-byte aligned) and are added to acc. Uniform control flow;
+// leaves s_lsum free.
 template <int K, typename XT, int P>
 __device__ __forceinline__ void long_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
-                                           const int (&lq)[P], const bool (&pv)[P], float (&acc)[P],
-                                           float (&cmp)[P], float *s_lsum, int nlsum)
+                                           const int (&lq)[P], const bool (&pv)[P], double (&acc)[P],
+                                           double *s_lsum, int nlsum)
 {
     if (A.L == 0) return;
     const int lb = cload(A.glong, u0 >> 6), le = cload(A.glong, ((u0 + rows - 1) >> 6) + 1);
     if (lb >= le) return;
     const int tid = threadIdx.x;
-    for (int q = tid; q < nlsum; q += kWG) s_lsum[q] = 0.0f;
+    for (int q = tid; q < nlsum; q += kWG) s_lsum[q] = 0.0;
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
     for (int l = lb + wave; l < le; l += kWaves) {
@@ -353,31 +340,31 @@ __device__ __forceinline__ void long_phase(const MatView &A, const XT *__restric
         if (row < u0 || row >= u0 + rows) continue;
         const int j0 = cload(A.lptr, l), j1 = cload(A.lptr, l + 1);
         for (int c = 0; c < K; c++) {
-            float part = 0.0f, pe = 0.0f;
+            double part = 0.0;
             for (int j = j0 + lane; j < j1; j += 64) {
                 const int v = A.cols[j];
                 const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
-                cadd(part, pe, (v & 1) ? xv : -xv);
+                accum(part, (v & 1) ? xv : -xv);
             }
-            part = wave_sum(part + pe);
+            part = wave_sum_d(part);
             if (lane == 0) s_lsum[(row - u0) * K + c] = part;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < P; p++)
-        if (pv[p]) cadd(acc[p], cmp[p], s_lsum[lq[p]]);
+        if (pv[p]) acc[p] += s_lsum[lq[p]];
     __syncthreads();
 }
 
 // Canonical per-(row, column) order, shared by every kernel below, with
-// compensated accumulation (cadd) into (acc, cmp) and y = acc + cmp:
+// FP64 accumulation (accum) into acc and y = (float)acc:
 //   diagonal terms (shape order, offset order);
 //   short-row entries in stored order, one lane, serially;
 //   the long-row wavefront sum (long rows have no short entries).
 template <int K, typename XT, int P>
 __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
-                                         int ldy, int u0, float *s_val, int capv, int *s_start, float *s_lsum)
+                                         int ldy, int u0, float *s_val, int capv, int *s_start, double *s_lsum)
 {
     constexpr int TRW = (kWG * P) / K;
     constexpr int NP = TRW * K;
@@ -387,7 +374,7 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
     // 1. row extents
     for (int r = tid; r <= TRW; r += kWG) s_start[r] = row_start(A, min(u0 + r, A.M));
 
-    float acc[P], cmp[P];
+    double acc[P];
     int prow[P], pcol[P], lq[P];
     bool pv[P];
 #pragma unroll
@@ -397,12 +384,11 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
         pcol[p] = q - prow[p] * K;
         pv[p] = q < NP && prow[p] < rows;
         lq[p] = q < NP ? q : 0;
-        acc[p] = 0.0f;
-        cmp[p] = 0.0f;
+        acc[p] = 0.0;
     }
 
     // 2. diagonal segments
-    diag_phase<K, XT, P>(A, X, ldx, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+    diag_phase<K, XT, P>(A, X, ldx, u0, rows, prow, pcol, pv, acc, s_val, capv);
     __syncthreads();
     const int ebase = s_start[0];
     const int E = s_start[TRW] - ebase;
@@ -418,21 +404,20 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
             if (!pv[p]) continue;
             const int lo = max(s_start[prow[p]] - ebase, e0);
             const int hi = min(s_start[prow[p] + 1] - ebase, e0 + ne);
-            float a = acc[p], e = cmp[p];
-            for (int j = lo; j < hi; j++) cadd(a, e, s_val[(j - e0) * K + pcol[p]]);
+            double a = acc[p];
+            for (int j = lo; j < hi; j++) accum(a, s_val[(j - e0) * K + pcol[p]]);
             acc[p] = a;
-            cmp[p] = e;
         }
         __syncthreads();
     }
 
     // 4. long rows: one wavefront per row
-    long_phase<K, XT, P>(A, X, ldx, u0, rows, lq, pv, acc, cmp, s_lsum, NP);
+    long_phase<K, XT, P>(A, X, ldx, u0, rows, lq, pv, acc, s_lsum, NP);
 
     // 5. store
 #pragma unroll
     for (int p = 0; p < P; p++)
-        if (pv[p]) Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]] = acc[p] + cmp[p];
+        if (pv[p]) Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]] = (float)acc[p];
 }
 
 // ---- single right-hand side: lane-strided row kernel ----
@@ -444,15 +429,15 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
 // two scalar loads (tiles are 64-row aligned). All loads of a phase are
 // issued before the first use. Same canonical order as run_tile. Column c of
 // X is X[c * ldx], row r of Y is Y[r * ldy] (1 and 1 for a plain vector).
-// LDS: s_val[capv] | s_lsum[TRW] (LONG) | s_win[TRW] (FAST).
+// LDS: s_val[capv] | s_lsum[TRW] doubles (LONG) | s_win[TRW] (FAST).
 template <typename XT, int RPT, int OW, bool FAST, bool LONG>
 __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
                                           int ldy, int t, float *smem, int capv)
 {
     constexpr int TRW = kWG * RPT;
     float *s_val = smem;
-    float *s_lsum = smem + capv;
-    float *s_win = smem + capv + (LONG ? TRW : 0);
+    double *s_lsum = (double *)(smem + lsum_at(capv));
+    float *s_win = smem + lsum_at(capv) + (LONG ? 2 * TRW : 0);
     const int tid = threadIdx.x, lane = tid & 63;
     const uint8_t *ro8 = (const uint8_t *)A.roff;
     const uint16_t *ro16 = (const uint16_t *)A.roff;
@@ -467,14 +452,13 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
         constexpr int U = 4;
         int prow[RPT], pcol[RPT], lq[RPT], endv[RPT], gbv[RPT];
         bool pv[RPT];
-        float acc[RPT], cmp[RPT];
+        double acc[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             prow[j] = j * kWG + tid;
             pcol[j] = 0;
             lq[j] = prow[j];
-            acc[j] = 0.0f;
-            cmp[j] = 0.0f;
+            acc[j] = 0.0;
             pv[j] = u0 + prow[j] < A.M;
         }
         // FAST (one shape, one offset): x[w0 + i], i < TRW, holds this tile's
@@ -506,11 +490,11 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
                 s_win[prow[j]] = xw[j];
                 const int r = u0 + prow[j], c = w0 + prow[j];
                 if (pv[j] && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs))
-                    cadd(acc[j], cmp[j], xw[j]);
+                    accum(acc[j], xw[j]);
             }
             __syncthreads();
         } else {
-            diag_phase<1, XT, RPT>(A, X, ldx, u0, rows, prow, pcol, pv, acc, cmp, s_val, capv);
+            diag_phase<1, XT, RPT>(A, X, ldx, u0, rows, prow, pcol, pv, acc, s_val, capv);
         }
 
         // a row starts where its left neighbour ends (DPP wave_shr:1); lane 0
@@ -553,20 +537,19 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
 #pragma unroll
             for (int j = 0; j < RPT; j++) {
                 const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
-                float a = acc[j], e = cmp[j];
-                for (int k = lo; k < hi; k++) cadd(a, e, s_val[k - e0]);
+                double a = acc[j];
+                for (int k = lo; k < hi; k++) accum(a, s_val[k - e0]);
                 acc[j] = a;
-                cmp[j] = e;
             }
             __syncthreads();
         }
 
-        if constexpr (LONG) long_phase<1, XT, RPT>(A, X, ldx, u0, rows, lq, pv, acc, cmp, s_lsum, TRW);
+        if constexpr (LONG) long_phase<1, XT, RPT>(A, X, ldx, u0, rows, lq, pv, acc, s_lsum, TRW);
 
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             if (!pv[j]) continue;
-            Y[(size_t)(u0 + prow[j]) * ldy] = acc[j] + cmp[j];
+            Y[(size_t)(u0 + prow[j]) * ldy] = (float)acc[j];
         }
     }
 }
@@ -608,7 +591,8 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
 // independent: no LDS staging; only long rows (one wavefront each, the fixed
 // butterfly) meet in LDS. Same canonical per-(row, column) order as every
 // other kernel. Requires X, Y, ldx and ldy aligned to CPL elements.
-// LDS: the x window [(TRW + kWideHalo) * K], reused for the long-row sums.
+// LDS: the x window [(TRW + kWideHalo) * K] floats, reused for the long-row
+// sums (TRW * K doubles; the launch sizes LDS for the larger of the two).
 constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
 
 template <int K, typename XT, int RI, bool LONG>
@@ -627,7 +611,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
     const uint16_t *ro16 = (const uint16_t *)A.roff;
     {
         const int u0 = t * TRW;
-        float acc[P], cmp[P];
+        double acc[P];
         int lq[P];
         bool pv[P];
         int rs[RI], re[RI];
@@ -643,8 +627,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             re[i] = valid ? gb + end : rs[i];
 #pragma unroll
             for (int m = 0; m < CPL; m++) {
-                acc[i * CPL + m] = 0.0f;
-                cmp[i * CPL + m] = 0.0f;
+                acc[i * CPL + m] = 0.0;
                 lq[i * CPL + m] = (GPW * i + g) * K + c0 + m;
                 pv[i * CPL + m] = valid;
             }
@@ -683,59 +666,44 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     }
                 }
                 __syncthreads();
-                for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
-                    const int n1 = min(n0 + kDiagBlock, sh.oe);
-                    float bs[P];
+                for (int n = sh.ob; n < sh.oe; n++) {
+                    const int o = cload(A.offs, n);
 #pragma unroll
-                    for (int p = 0; p < P; p++) bs[p] = 0.0f;
-                    for (int n = n0; n < n1; n++) {
-                        const int o = cload(A.offs, n);
-#pragma unroll
-                        for (int i = 0; i < RI; i++) {
-                            const int r = u0 + GPW * i + g;
-                            const int c = r + shift + o;
-                            if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
-                                const float *sp = smem + (c - w0) * K + c0;
-                                if constexpr (CPL == 4) {
-                                    const float4 q = *(const float4 *)sp;
-                                    bs[i * 4] += q.x, bs[i * 4 + 1] += q.y, bs[i * 4 + 2] += q.z, bs[i * 4 + 3] += q.w;
-                                } else {
-                                    const float2 q = *(const float2 *)sp;
-                                    bs[i * 2] += q.x, bs[i * 2 + 1] += q.y;
-                                }
+                    for (int i = 0; i < RI; i++) {
+                        const int r = u0 + GPW * i + g;
+                        const int c = r + shift + o;
+                        if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
+                            const float *sp = smem + (c - w0) * K + c0;
+                            if constexpr (CPL == 4) {
+                                const float4 q = *(const float4 *)sp;
+                                accum(acc[i * 4], q.x), accum(acc[i * 4 + 1], q.y);
+                                accum(acc[i * 4 + 2], q.z), accum(acc[i * 4 + 3], q.w);
+                            } else {
+                                const float2 q = *(const float2 *)sp;
+                                accum(acc[i * 2], q.x), accum(acc[i * 2 + 1], q.y);
                             }
                         }
                     }
-#pragma unroll
-                    for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
                 }
                 __syncthreads();
                 continue;
             }
-            for (int n0 = sh.ob; n0 < sh.oe; n0 += kDiagBlock) {
-                const int n1 = min(n0 + kDiagBlock, sh.oe);
-                float bs[P];
+            for (int n = sh.ob; n < sh.oe; n++) {
+                const int o = cload(A.offs, n);
+                float xv[RI][CPL];
+                bool in[RI];
 #pragma unroll
-                for (int p = 0; p < P; p++) bs[p] = 0.0f;
-                for (int n = n0; n < n1; n++) {
-                    const int o = cload(A.offs, n);
-                    float xv[RI][CPL];
-                    bool in[RI];
-#pragma unroll
-                    for (int i = 0; i < RI; i++) {
-                        const int r = u0 + GPW * i + g;
-                        const int c = r - sh.xs + sh.ys + o;
-                        in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
-                        if (in[i]) load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
-                    }
-#pragma unroll
-                    for (int i = 0; i < RI; i++)
-                        if (in[i])
-#pragma unroll
-                            for (int m = 0; m < CPL; m++) bs[i * CPL + m] += xv[i][m];
+                for (int i = 0; i < RI; i++) {
+                    const int r = u0 + GPW * i + g;
+                    const int c = r - sh.xs + sh.ys + o;
+                    in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
+                    if (in[i]) load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
                 }
 #pragma unroll
-                for (int p = 0; p < P; p++) cadd(acc[p], cmp[p], bs[p]);
+                for (int i = 0; i < RI; i++)
+                    if (in[i])
+#pragma unroll
+                        for (int m = 0; m < CPL; m++) accum(acc[i * CPL + m], xv[i][m]);
             }
         }
 
@@ -755,11 +723,12 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     if (j0 + u < re[i])
 #pragma unroll
                         for (int m = 0; m < CPL; m++)
-                            cadd(acc[i * CPL + m], cmp[i * CPL + m], (v[u] & 1) ? xv[u][m] : -xv[u][m]);
+                            accum(acc[i * CPL + m], (v[u] & 1) ? xv[u][m] : -xv[u][m]);
             }
         }
 
-        if constexpr (LONG) long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, cmp, smem, TRW * K);
+        if constexpr (LONG)
+            long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, (double *)smem, TRW * K);
 
 #pragma unroll
         for (int i = 0; i < RI; i++) {
@@ -767,17 +736,17 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             if (r >= A.M) continue;
             float *yp = Y + (size_t)r * ldy + c0;
             if constexpr (CPL == 4)
-                *(float4 *)yp = make_float4(acc[i * 4] + cmp[i * 4], acc[i * 4 + 1] + cmp[i * 4 + 1],
-                                            acc[i * 4 + 2] + cmp[i * 4 + 2], acc[i * 4 + 3] + cmp[i * 4 + 3]);
-            else *(float2 *)yp = make_float2(acc[i * 2] + cmp[i * 2], acc[i * 2 + 1] + cmp[i * 2 + 1]);
+                *(float4 *)yp = make_float4((float)acc[i * 4], (float)acc[i * 4 + 1], (float)acc[i * 4 + 2],
+                                            (float)acc[i * 4 + 3]);
+            else *(float2 *)yp = make_float2((float)acc[i * 2], (float)acc[i * 2 + 1]);
         }
     }
 }
 
-// LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
+// LDS: s_val[capv] | s_lsum[256*P] doubles | s_start[TRW+1]
 __host__ __device__ constexpr size_t lds_bytes(int capv, int P, int trw)
 {
-    return ((size_t)capv + (size_t)kWG * P + (size_t)trw + 1) * 4;
+    return (size_t)lsum_at(capv) * 4 + (size_t)kWG * P * 8 + ((size_t)trw + 1) * 4;
 }
 
 }  // namespace tcd

@@ -21,7 +21,10 @@ int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
     const int ntiles = (A.M + TRW - 1) / TRW;
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
-    const size_t lds = (A.S > 0 || A.L > 0) ? (size_t)(TRW + kWideHalo) * K * 4 : 0;
+    // the x window (floats), reused for the long-row sums (doubles)
+    const size_t lds = (A.S > 0 || A.L > 0)
+                           ? std::max((size_t)(TRW + kWideHalo) * K * 4, A.L > 0 ? (size_t)TRW * K * 8 : 0)
+                           : 0;
     if (A.L > 0)
         spmv_wide_kernel<K, XT, RI, true><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
     else
