@@ -373,7 +373,7 @@ def main():
     log(f"generated R-MAT 2^{args.scale}: nnz={nnz} in {time.time() - t0:.1f}s")
     t0 = time.time()
     fmt, _ = T.partition_shapes(n, n, rp, col)
-    log(f"partition (GPU thumbnail + host): {time.time() - t0:.2f}s S={fmt.dia_shapes} "
+    log(f"partition (device build): {time.time() - t0:.2f}s S={fmt.dia_shapes} "
         f"D={fmt.num_negs} dense={fmt.dense_nnz} Z={fmt.zero_elements} P={fmt.sparse_nnz}")
     A = T.Matrix(fmt, nnz=nnz)
     st = A.stats()
