@@ -30,9 +30,9 @@ class OrcTaiChiNew(C.Structure):
 
 
 def build():
-    subprocess.check_call(["make", "-s", "-C", _HERE, "all"])
+    subprocess.check_call(["make", "-B", "-s", "-C", _HERE, "all"])
     if os.path.isdir("/root/reference"):
-        subprocess.check_call(["make", "-s", "-C", _HERE, "ref"])
+        subprocess.check_call(["make", "-B", "-s", "-C", _HERE, "ref"])
 
 
 def lib():

@@ -89,9 +89,10 @@ class Request(C.Structure):
 
 
 def build():
-    """Compile libtaichi_amd.so and taichi_spmv for gfx950 in-tree."""
+    """Compile libtaichi_amd.so and taichi_spmv for gfx950 in-tree (always a full build)."""
     jobs = str(min(16, os.cpu_count() or 4))
-    subprocess.check_call(["make", "-s", "-C", os.path.join(_HERE, "csrc"), "-j", jobs, "all"])
+    # -B: every call compiles every source, also when the library is up to date
+    subprocess.check_call(["make", "-B", "-s", "-C", os.path.join(_HERE, "csrc"), "-j", jobs, "all"])
 
 
 def lib():
