@@ -214,7 +214,12 @@ typedef struct tc_config {
     int flags;           /* bit 0: reserved (ignored). bit 1: at k = 1 use the
                             one-row-per-lane kernel, not quad rows (quad rows
                             apply to 512/1024-row tiles of matrices with at
-                            most one shape of one offset) */
+                            most one shape of one offset). bit 2: k > 1 on the
+                            staged tile kernel. bit 3 (in the config of width
+                            4 or 8): no masked chunks -- 3, 5, 6, 7 columns
+                            then run as narrower chunks (or the staged
+                            kernel) instead of one masked pass of that
+                            width */
 } tc_config;
 int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg);
 int tc_matrix_get_config(const tc_matrix *A, int k, int dtype, tc_config *cfg);

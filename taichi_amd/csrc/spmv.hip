@@ -376,8 +376,20 @@ int tc_spmv(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, 
     // depend on the chunking
     for (int c0 = 0; c0 < k;) {
         const int K = pick_k(k - c0);
+        const void *xc = (const char *)X + c0 * es;
+        float *yc = Y + c0;
+        const int W = masked_width(K);
+        if (W) {
+            const tc_config cw = get_cfg(A, W, dtype);
+            if (!(cw.flags & kNoMask) && wide_applies(W, xc, ldx, yc, ldy, (int)es, cw)) {
+                rc = launch_wide(A->v, xc, ldx, yc, ldy, dtype, W, cw, s, K);
+                if (rc != TC_OK) return rc;
+                c0 += K;
+                continue;
+            }
+        }
         const tc_config c = get_cfg(A, K, dtype);
-        rc = launch_cfg(A, (const char *)X + c0 * es, ldx, dtype, Y + c0, ldy, K, c, s);
+        rc = launch_cfg(A, xc, ldx, dtype, yc, ldy, K, c, s);
         if (rc != TC_OK) return rc;
         c0 += K;
     }

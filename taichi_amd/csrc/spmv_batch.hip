@@ -27,6 +27,7 @@ struct WorkDesc {
     int ldx, ldy;
     int unit_begin;
     int fast;  // one shape with one offset: the rows tile's FAST path
+    int kv;    // columns that exist (a masked wide chunk has kv < its width)
 };
 
 __device__ __forceinline__ WorkDesc load_desc(const WorkDesc *D, int d)
@@ -113,15 +114,15 @@ __global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restr
 // K > 1 chunks whose X and Y rows split into aligned vectors: wide-rows tiles
 constexpr int kBatchRI = 1;  // one row per lane group: 42-56 VGPRs, full occupancy
 
-template <int K, typename XT>
+template <int K, typename XT, bool MASK>
 __global__ __launch_bounds__(kWG) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
                                                          int nd, int total)
 {
     extern __shared__ float smem[];
     __shared__ int s_ub[kBatchLdsDesc];
     batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
-        if (w.A.L > 0) wide_tile<K, XT, kBatchRI, true>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem);
-        else wide_tile<K, XT, kBatchRI, false>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem);
+        if (w.A.L > 0) wide_tile<K, XT, kBatchRI, true, MASK>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem, w.kv);
+        else wide_tile<K, XT, kBatchRI, false, MASK>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t, smem, w.kv);
     });
 }
 
@@ -143,31 +144,40 @@ size_t batch_desc_bytes(int max_requests)
 }
 
 // Batch classes: (kernel, chunk width, x type). Kernel 0: rows tiles (K = 1);
-// 1: wide-rows tiles (K > 1, aligned); 2: staged tiles (K > 1, otherwise).
-constexpr int kBatchClasses = 3 * 4 * 3;
+// 1: wide-rows tiles (K > 1, aligned); 2: staged tiles (K > 1, otherwise);
+// 3: masked wide-rows tiles (fewer columns than the width, aligned).
+constexpr int kBatchClasses = 4 * 4 * 3;
 inline int batch_class(int kern, int ki, int dt) { return (kern * 4 + ki) * 3 + dt; }
 
 int batch_tile_rows(int kern, int K)
 {
     if (kern == 0) return kWG * kBatchRPT;
-    if (kern == 1) return wide_rows(K, kBatchRI);
+    if (kern == 1 || kern == 3) return wide_rows(K, kBatchRI);
     return (kWG * kBatchP) / K;
 }
 
 template <typename XT>
 void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int nd, int units, hipStream_t s)
 {
-    const int grid = units;  // one workgroup per tile: the dispatcher balances chunks of any size
+    // one workgroup per tile: the dispatcher balances chunks of any size (a
+    // grid of 8 workgroups per CU walking runs of tiles measured slower)
+    const int grid = units;
     if (kern == 0) {
         constexpr int TRW = kWG * kBatchRPT;
         const size_t lds = ((size_t)lsum_at(kBatchCapv) + 3 * TRW) * 4;  // s_val | long sums (doubles) | window
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
-    } else if (kern == 1) {
+    } else if (kern == 1 || kern == 3) {
         const size_t lds = std::max((size_t)(wide_rows(K, kBatchRI) + kWideHalo) * K * 4,
                                     (size_t)wide_rows(K, kBatchRI) * K * 8);
-        if (K == 2) batch_wide_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
-        else if (K == 4) batch_wide_kernel<4, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
-        else batch_wide_kernel<8, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+        if (kern == 3) {
+            if (K == 2) batch_wide_kernel<2, XT, true><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+            else if (K == 4) batch_wide_kernel<4, XT, true><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+            else batch_wide_kernel<8, XT, true><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+        } else {
+            if (K == 2) batch_wide_kernel<2, XT, false><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+            else if (K == 4) batch_wide_kernel<4, XT, false><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+            else batch_wide_kernel<8, XT, false><<<grid, kWG, lds, s>>>(D, UB, nd, units);
+        }
     } else {
         const size_t lds = lds_bytes(kBatchCapv, kBatchP, (kWG * kBatchP) / K);
         if (K == 2) batch_tile_kernel<2, XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
@@ -176,15 +186,20 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
     }
 }
 
-// chunk c0..c0+K of request r: its batch class
-int chunk_class(const tc_request &r, int ki, int K, int c0, int dt)
+// chunk c0..c0+K of request r (kv of its columns exist): its batch class
+bool chunk_aligned(const tc_request &r, int K, int c0)
 {
-    if (K == 1) return batch_class(0, ki, dt);
     const int es = elem_size(r.dtype), cpl = std::min(K, 4);
     const uintptr_t xa = (uintptr_t)r.X + (uintptr_t)c0 * es, ya = (uintptr_t)(r.Y + c0);
-    const bool aligned = r.ldx % cpl == 0 && r.ldy % cpl == 0 && xa % ((uintptr_t)cpl * es) == 0 &&
-                         ya % ((uintptr_t)cpl * 4) == 0;
-    return batch_class(aligned ? 1 : 2, ki, dt);
+    return r.ldx % cpl == 0 && r.ldy % cpl == 0 && xa % ((uintptr_t)cpl * es) == 0 &&
+           ya % ((uintptr_t)cpl * 4) == 0;
+}
+
+int chunk_class(const tc_request &r, int ki, int K, int c0, int dt, int kv)
+{
+    if (kv < K) return batch_class(3, ki, dt);  // masked: chosen only when aligned
+    if (K == 1) return batch_class(0, ki, dt);
+    return batch_class(chunk_aligned(r, K, c0) ? 1 : 2, ki, dt);
 }
 
 // Fills the chunk descriptors of n requests into host_desc (at least
@@ -194,17 +209,30 @@ int chunk_class(const tc_request &r, int ki, int K, int c0, int dt)
 int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s)
 {
     static const int kChunkK[4] = {1, 2, 4, 8};
-    // chunks of a request: 8s, then 4, 2, 1 as k's bits say; calls
-    // f(ki, K, c0) for each, widest first
-    auto chunks = [](int k, auto &&f) {
+    // chunks of a request: 8s; then the rest (1..7 columns) as one masked
+    // chunk of the next width when its rows are aligned and the mask pays,
+    // else 4, 2, 1 as its bits say. Calls f(ki, K, c0, kv) for each, widest
+    // first.
+    auto chunks = [](const tc_request &r, auto &&f) {
+        const int k = r.k;
         int c0 = 0;
         while (k - c0 >= 8) {
-            f(3, 8, c0);
+            f(3, 8, c0, 8);
             c0 += 8;
+        }
+        const int rest = k - c0;
+        // 5 or 6 of 8 columns waste a quarter or more of the arithmetic,
+        // which costs more than the second pass saves on matrices with many
+        // dense diagonals (measured): those split into 4 + 1 and 4 + 2
+        const bool diag_heavy = r.A->st.num_negs > 2 && (rest == 5 || rest == 6);
+        const int W = rest > 0 && !diag_heavy ? masked_width(rest) : 0;
+        if (W && chunk_aligned(r, W, c0) && !(get_cfg(r.A, W, r.dtype).flags & kNoMask)) {
+            f(W == 2 ? 1 : W == 4 ? 2 : 3, W, c0, rest);
+            return;
         }
         for (int ki = 2; ki >= 0; ki--)
             if ((k - c0) & (1 << ki)) {
-                f(ki, 1 << ki, c0);
+                f(ki, 1 << ki, c0, 1 << ki);
                 c0 += 1 << ki;
             }
     };
@@ -216,8 +244,8 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
         if (rc != TC_OK) return rc;
         if (r.k > kMaxBatchK) return tc::fail(TC_ERR_UNSUPPORTED, "batch: k=%d > %d", r.k, kMaxBatchK);
         const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
-        chunks(r.k, [&](int ki, int K, int c0) {
-            cls_count[chunk_class(r, ki, K, c0, dt)]++;
+        chunks(r, [&](int ki, int K, int c0, int kv) {
+            cls_count[chunk_class(r, ki, K, c0, dt, kv)]++;
             nchunks++;
         });
     }
@@ -236,8 +264,8 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
         const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
         const MatView &v = r.A->v;
         const bool fast = v.S == 1 && v.sh0.oe - v.sh0.ob == 1;
-        chunks(r.k, [&](int ki, int K, int c0) {
-            const int cl = chunk_class(r, ki, K, c0, dt);
+        chunks(r, [&](int ki, int K, int c0, int kv) {
+            const int cl = chunk_class(r, ki, K, c0, dt, kv);
             WorkDesc &w = D[fill[cl]];
             w.A = v;
             w.X = (const char *)r.X + (size_t)c0 * elem_size(r.dtype);
@@ -245,6 +273,7 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
             w.ldx = r.ldx;
             w.ldy = r.ldy;
             w.fast = fast ? 1 : 0;
+            w.kv = kv;
             const int trw = batch_tile_rows(cl / 12, K);
             w.unit_begin = (int)units[cl];
             UB[fill[cl]] = w.unit_begin;

@@ -66,6 +66,19 @@ inline bool wide_k(int K) { return K == 2 || K == 4 || K == 8 || K == 16 || K ==
 // rows of a wide-rows tile with RI rows per lane group
 inline int wide_rows(int K, int RI) { return (kWG / (K / std::min(K, 4))) * RI; }
 
+// A column count with no wide kernel of its own (3, 5, 6, 7) runs as one
+// masked pass of this wide width (0: none) when X and Y rows split into
+// aligned vectors of it: one pass over the indices instead of the staged
+// kernel or two or three narrower chunks. (A single strided column measured
+// slower masked than on the rows or staged kernels, so it is not masked.)
+inline int masked_width(int K)
+{
+    if (K == 3) return 4;
+    if (K >= 5 && K <= 7) return 8;
+    return 0;
+}
+
+constexpr int kNoMask = 8;   // config flag (of the wide width's config): no masked chunks
 constexpr int kQuadOff = 2;  // config flag: use the one-row-per-lane kernel at k = 1
 
 // P from a rows_per_tile request for column count K (1, 2 or 4)
@@ -95,7 +108,8 @@ int launch_staged(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
                   int grid, hipStream_t s);
 // wide rows, K in {2, 4, 8, 16, 32, 64}: whether they apply, and the launch (spmv_wide.hip)
 bool wide_applies(int K, const void *X, int ldx, const float *Y, int ldy, int esize, const tc_config &c);
+// kv < K: only the first kv columns exist (a masked chunk, K <= 8)
 int launch_wide(const MatView &A, const void *X, int ldx, float *Y, int ldy, int dtype, int K, const tc_config &c,
-                hipStream_t s);
+                hipStream_t s, int kv = 0);
 
 }  // namespace tcd

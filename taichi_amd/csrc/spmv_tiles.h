@@ -321,12 +321,13 @@ __device__ __forceinline__ void diag_phase(const MatView &A, const XT *__restric
 
 // Long rows of the tile: one wavefront each, FP64 partial sums strided over
 // the 64 lanes and reduced by wave_sum_d; results land in s_lsum[local_row*K
-// + c] (doubles, 8-byte aligned) and are added to acc. Uniform control flow;
-// leaves s_lsum free.
+// + c] (doubles, 8-byte aligned) and are added to acc. Only columns c < kc
+// are read (kc < K for a masked chunk; the others stay 0). Uniform control
+// flow; leaves s_lsum free.
 template <int K, typename XT, int P>
 __device__ __forceinline__ void long_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
                                            const int (&lq)[P], const bool (&pv)[P], double (&acc)[P],
-                                           double *s_lsum, int nlsum)
+                                           double *s_lsum, int nlsum, int kc = K)
 {
     if (A.L == 0) return;
     const int lb = cload(A.glong, u0 >> 6), le = cload(A.glong, ((u0 + rows - 1) >> 6) + 1);
@@ -339,7 +340,7 @@ __device__ __forceinline__ void long_phase(const MatView &A, const XT *__restric
         const int row = cload(A.lrow, l);
         if (row < u0 || row >= u0 + rows) continue;
         const int j0 = cload(A.lptr, l), j1 = cload(A.lptr, l + 1);
-        for (int c = 0; c < K; c++) {
+        for (int c = 0; c < kc; c++) {
             double part = 0.0;
             for (int j = j0 + lane; j < j1; j += 64) {
                 const int v = A.cols[j];
@@ -582,6 +583,24 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
     }
 }
 
+// x row segment of a masked chunk: elements at or past nv are returned as 0.
+// The whole Q-wide vector is loaded (no divergence across the lane group):
+// it lies inside the row's ldx span (ldx is a multiple of Q and the lane's
+// first column is below kv <= ldx), except in the last row of X, which may
+// end at column kv -- there (last) only the first nv elements are read.
+template <typename XT, int Q>
+__device__ __forceinline__ void load_qm(const XT *__restrict__ p, int nv, bool last, float (&o)[Q])
+{
+    if (!last || nv >= Q) {
+        load_q<XT, Q>(p, o);
+#pragma unroll
+        for (int m = 0; m < Q; m++) o[m] = m < nv ? o[m] : 0.0f;
+    } else {
+#pragma unroll
+        for (int m = 0; m < Q; m++) o[m] = m < nv ? load_x<XT>(p, m) : 0.0f;
+    }
+}
+
 // ---- k > 1: wide rows ----
 // A row belongs to a group of LPR = K/CPL lanes; lane q of the group owns the
 // CPL = min(K, 4) consecutive columns q*CPL.., so each x row segment the row
@@ -591,13 +610,18 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
 // independent: no LDS staging; only long rows (one wavefront each, the fixed
 // butterfly) meet in LDS. Same canonical per-(row, column) order as every
 // other kernel. Requires X, Y, ldx and ldy aligned to CPL elements.
+// MASK: only the first kv < K columns exist; a lane computes all its
+// columns but keeps only those below kv (load_qm zeroes the rest; stores are
+// masked). A masked chunk runs k = 3, 5, 6, 7 as one pass of width 4 or 8
+// instead of 2 + 1, 4 + 1, 4 + 2, 4 + 2 + 1 passes. Needs kv > K/2 (so a
+// lane's vector stays inside its X row, or X's last row is handled apart).
 // LDS: the x window [(TRW + kWideHalo) * K] floats, reused for the long-row
 // sums (TRW * K doubles; the launch sizes LDS for the larger of the two).
 constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
 
-template <int K, typename XT, int RI, bool LONG>
+template <int K, typename XT, int RI, bool LONG, bool MASK = false>
 __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
-                                          int ldy, int t, float *smem)
+                                          int ldy, int t, float *smem, int kv = K)
 {
     constexpr int CPL = K < 4 ? K : 4;
     constexpr int LPR = K / CPL;
@@ -607,6 +631,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
     constexpr int U = 4;            // remainder entries in flight per lane
     const int tid = threadIdx.x;
     const int g = tid / LPR, c0 = (tid % LPR) * CPL;
+    const int nv = MASK ? min(max(kv - c0, 0), CPL) : CPL;  // this lane's columns that exist
     const uint8_t *ro8 = (const uint8_t *)A.roff;
     const uint16_t *ro16 = (const uint16_t *)A.roff;
     {
@@ -647,18 +672,22 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             const int w0 = max(sh.ys, ra + shift + sh.omin);
             const int w1 = min(sh.ys + side, rb - 1 + shift + sh.omax + 1);
             if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
-                const int nv = (w1 - w0) * LPR;  // CPL-wide vectors of the window
-                for (int b = 0; b < nv; b += kWG * 4) {
+                const int nw = (w1 - w0) * LPR;  // CPL-wide vectors of the window
+                for (int b = 0; b < nw; b += kWG * 4) {
                     float tv[4][CPL];
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
-                        const int v = min(b + u * kWG + tid, nv - 1);
-                        load_q<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL, tv[u]);
+                        const int v = min(b + u * kWG + tid, nw - 1);
+                        const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
+                        if constexpr (MASK)
+                            load_qm<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == A.N - 1,
+                                             tv[u]);
+                        else load_q<XT, CPL>(xp, tv[u]);
                     }
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int v = b + u * kWG + tid;
-                        if (v < nv) {
+                        if (v < nw) {
                             float *d = smem + (v / LPR) * K + (v % LPR) * CPL;
                             if constexpr (CPL == 4) *(float4 *)d = make_float4(tv[u][0], tv[u][1], tv[u][2], tv[u][3]);
                             else *(float2 *)d = make_float2(tv[u][0], tv[u][1]);
@@ -697,7 +726,10 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     const int r = u0 + GPW * i + g;
                     const int c = r - sh.xs + sh.ys + o;
                     in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
-                    if (in[i]) load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                    if (in[i]) {
+                        if constexpr (MASK) load_qm<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xv[i]);
+                        else load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < RI; i++)
@@ -717,7 +749,11 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 for (int u = 0; u < U; u++) v[u] = j0 + u < re[i] ? A.cols[j0 + u] : 1;
 #pragma unroll
                 for (int u = 0; u < U; u++)
-                    if (j0 + u < re[i]) load_q<XT, CPL>(X + (size_t)(v[u] >> 1) * ldx + c0, xv[u]);
+                    if (j0 + u < re[i]) {
+                        if constexpr (MASK)
+                            load_qm<XT, CPL>(X + (size_t)(v[u] >> 1) * ldx + c0, nv, (v[u] >> 1) == A.N - 1, xv[u]);
+                        else load_q<XT, CPL>(X + (size_t)(v[u] >> 1) * ldx + c0, xv[u]);
+                    }
 #pragma unroll
                 for (int u = 0; u < U; u++)
                     if (j0 + u < re[i])
@@ -728,13 +764,20 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
         }
 
         if constexpr (LONG)
-            long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, (double *)smem, TRW * K);
+            long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, (double *)smem, TRW * K,
+                                 MASK ? kv : K);
 
 #pragma unroll
         for (int i = 0; i < RI; i++) {
             const int r = u0 + GPW * i + g;
             if (r >= A.M) continue;
             float *yp = Y + (size_t)r * ldy + c0;
+            if (MASK && nv < CPL) {
+#pragma unroll
+                for (int m = 0; m < CPL; m++)
+                    if (m < nv) yp[m] = (float)acc[i * CPL + m];
+                continue;
+            }
             if constexpr (CPL == 4)
                 *(float4 *)yp = make_float4((float)acc[i * 4], (float)acc[i * 4 + 1], (float)acc[i * 4 + 2],
                                             (float)acc[i * 4 + 3]);

@@ -4,18 +4,19 @@
 
 namespace tcd {
 
-template <int K, typename XT, int RI, bool LONG>
+template <int K, typename XT, int RI, bool LONG, bool MASK>
 __global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
-                                                        float *__restrict__ Y, int ldy, int ntiles)
+                                                        float *__restrict__ Y, int ldy, int ntiles, int kv)
 {
     extern __shared__ float smem[];
     int tb, te;
     block_units(ntiles, &tb, &te);
-    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG>(A, X, ldx, Y, ldy, t, smem);
+    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG, MASK>(A, X, ldx, Y, ldy, t, smem, kv);
 }
 
+// kv < K: a masked chunk (only K = 2, 4, 8 are instantiated masked)
 template <int K, typename XT, int RI>
-int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int grid, hipStream_t s)
+int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int grid, int kv, hipStream_t s)
 {
     constexpr int TRW = (kWG / (K / (K < 4 ? K : 4))) * RI;
     const int ntiles = (A.M + TRW - 1) / TRW;
@@ -25,32 +26,40 @@ int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
     const size_t lds = (A.S > 0 || A.L > 0)
                            ? std::max((size_t)(TRW + kWideHalo) * K * 4, A.L > 0 ? (size_t)TRW * K * 8 : 0)
                            : 0;
-    if (A.L > 0)
-        spmv_wide_kernel<K, XT, RI, true><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
-    else
-        spmv_wide_kernel<K, XT, RI, false><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles);
+    const XT *x = (const XT *)X;
+    if constexpr (K <= 8) {
+        if (kv < K) {
+            if (A.L > 0) spmv_wide_kernel<K, XT, RI, true, true><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, kv);
+            else spmv_wide_kernel<K, XT, RI, false, true><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, kv);
+            HIPCHECK(hipGetLastError());
+            return TC_OK;
+        }
+    }
+    if (A.L > 0) spmv_wide_kernel<K, XT, RI, true, false><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, K);
+    else spmv_wide_kernel<K, XT, RI, false, false><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, K);
     HIPCHECK(hipGetLastError());
     return TC_OK;
 }
 
 template <int K, typename XT>
-int launch_wide_k(const MatView &A, const void *X, int ldx, float *Y, int ldy, const tc_config &c, hipStream_t s)
+int launch_wide_k(const MatView &A, const void *X, int ldx, float *Y, int ldy, const tc_config &c, int kv,
+                  hipStream_t s)
 {
-    if (c.rows_per_tile == wide_rows(K, 1)) return launch_wide_t<K, XT, 1>(A, X, ldx, Y, ldy, c.grid, s);
-    return launch_wide_t<K, XT, 2>(A, X, ldx, Y, ldy, c.grid, s);
+    if (c.rows_per_tile == wide_rows(K, 1)) return launch_wide_t<K, XT, 1>(A, X, ldx, Y, ldy, c.grid, kv, s);
+    return launch_wide_t<K, XT, 2>(A, X, ldx, Y, ldy, c.grid, kv, s);
 }
 
 template <typename XT>
-int launch_wide_x(const MatView &A, const void *X, int ldx, float *Y, int ldy, int K, const tc_config &c,
-                hipStream_t s)
+int launch_wide_x(const MatView &A, const void *X, int ldx, float *Y, int ldy, int K, const tc_config &c, int kv,
+                  hipStream_t s)
 {
     switch (K) {
-    case 2: return launch_wide_k<2, XT>(A, X, ldx, Y, ldy, c, s);
-    case 4: return launch_wide_k<4, XT>(A, X, ldx, Y, ldy, c, s);
-    case 8: return launch_wide_k<8, XT>(A, X, ldx, Y, ldy, c, s);
-    case 16: return launch_wide_k<16, XT>(A, X, ldx, Y, ldy, c, s);
-    case 32: return launch_wide_k<32, XT>(A, X, ldx, Y, ldy, c, s);
-    default: return launch_wide_k<64, XT>(A, X, ldx, Y, ldy, c, s);
+    case 2: return launch_wide_k<2, XT>(A, X, ldx, Y, ldy, c, kv, s);
+    case 4: return launch_wide_k<4, XT>(A, X, ldx, Y, ldy, c, kv, s);
+    case 8: return launch_wide_k<8, XT>(A, X, ldx, Y, ldy, c, kv, s);
+    case 16: return launch_wide_k<16, XT>(A, X, ldx, Y, ldy, c, K, s);
+    case 32: return launch_wide_k<32, XT>(A, X, ldx, Y, ldy, c, K, s);
+    default: return launch_wide_k<64, XT>(A, X, ldx, Y, ldy, c, K, s);
     }
 }
 
@@ -65,11 +74,13 @@ bool wide_applies(int K, const void *X, int ldx, const float *Y, int ldy, int es
 }
 
 int launch_wide(const MatView &A, const void *X, int ldx, float *Y, int ldy, int dtype, int K, const tc_config &c,
-                hipStream_t s)
+                hipStream_t s, int kv)
 {
-    if (dtype == TC_BF16) return launch_wide_x<bf16_t>(A, X, ldx, Y, ldy, K, c, s);
-    if (dtype == TC_F16) return launch_wide_x<f16_t>(A, X, ldx, Y, ldy, K, c, s);
-    return launch_wide_x<float>(A, X, ldx, Y, ldy, K, c, s);
+    if (kv <= 0 || kv > K) kv = K;
+    if (kv < K && K > 8) return tc::fail(TC_ERR_UNSUPPORTED, "masked wide chunks are 2, 4 or 8 columns wide");
+    if (dtype == TC_BF16) return launch_wide_x<bf16_t>(A, X, ldx, Y, ldy, K, c, kv, s);
+    if (dtype == TC_F16) return launch_wide_x<f16_t>(A, X, ldx, Y, ldy, K, c, kv, s);
+    return launch_wide_x<float>(A, X, ldx, Y, ldy, K, c, kv, s);
 }
 
 }  // namespace tcd

@@ -400,3 +400,17 @@ def test_masked_column_chunks(case, dev):
         torch.cuda.synchronize()
         for c in range(k):
             assert torch.equal(Yt[:, c], cols1[c]), (k, c)
+    # masked path over a buffer that ends at column k of X's last row, with
+    # NaN in every slot past column k: a masked chunk must not use them
+    for k in (1, 3, 5, 6, 7):
+        flat = torch.full(((N - 1) * 8 + k,), float("nan"), device=dev)
+        Xs = flat.as_strided((N, k), (8, 1))
+        Xs.copy_(X8[:, :k])
+        Ys = torch.full((M, 8), float("nan"), device=dev)[:, :k]
+        A.spmv(Xs, Ys)
+        Yb = torch.full((M, 8), float("nan"), device=dev)[:, :k]
+        T.spmv_batch([(A, Xs, Yb)])
+        torch.cuda.synchronize()
+        for c in range(k):
+            assert torch.equal(Ys[:, c], cols1[c]), (k, c)
+            assert torch.equal(Yb[:, c], cols1[c]), (k, c)
