@@ -252,6 +252,11 @@ typedef struct tc_server_config {
     int max_batch;            /* requests per launch */
     int streams;              /* launch slots that may run concurrently */
     int max_inflight;         /* requests on the GPU at once */
+    int launch;               /* how a batch is launched on its slot's stream:
+                                 0 (auto): requests of at least 32768 rows one
+                                 by one (tc_spmv), the others as one batched
+                                 launch group (tc_spmv_batch); 1: all batched;
+                                 2: all one by one. Same results either way. */
 } tc_server_config;
 typedef struct tc_server_stats {
     long long resident_bytes;

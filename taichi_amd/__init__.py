@@ -56,7 +56,7 @@ class Config(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_cap_bytes", C.c_longlong), ("max_batch", C.c_int), ("streams", C.c_int),
-                ("max_inflight", C.c_int)]
+                ("max_inflight", C.c_int), ("launch", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -635,9 +635,12 @@ def footprint(fmt, long_threshold=0):
 class Server:
     """Continuous-batching SpMV server over resident matrices (tc_server)."""
 
-    def __init__(self, hbm_cap_bytes=200 << 30, max_batch=64, streams=3, max_inflight=256):
+    def __init__(self, hbm_cap_bytes=200 << 30, max_batch=64, streams=3, max_inflight=256, launch=0):
+        """launch 0 (auto): requests of at least 32768 rows run as their own
+        launches, the rest as one batched launch group; 1: all batched; 2: all
+        one by one. Same results either way."""
         h = C.c_void_p()
-        cfg = ServerConfig(hbm_cap_bytes, max_batch, streams, max_inflight)
+        cfg = ServerConfig(hbm_cap_bytes, max_batch, streams, max_inflight, launch)
         check(lib().tc_server_create(C.byref(cfg), C.byref(h)), "server_create")
         self.handle = h
 

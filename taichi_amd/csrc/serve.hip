@@ -8,11 +8,12 @@
 //     allocation) fit under the configured HBM cap.
 //   * Continuous batching: a dispatcher thread takes whatever requests are
 //     pending (up to max_batch, and up to max_inflight requests on the GPU)
-//     as soon as one of `streams` launch slots is free, writes their work
-//     descriptors into that slot's pinned buffer and enqueues one persistent
-//     launch on that slot's stream. Requests that arrive while batches run
-//     wait only for the next free slot, and batches on different streams
-//     overlap on the device.
+//     as soon as one of `streams` launch slots is free and enqueues them on
+//     that slot's stream: requests of at least kAloneRows rows as their own
+//     launches (tc_spmv; their kernels dwarf a launch), the others as one
+//     batched launch group whose work descriptors go through the slot's
+//     pinned buffer. Requests that arrive while batches run wait only for the
+//     next free slot, and batches on different streams overlap on the device.
 //   * A completion thread polls the oldest in-flight batch's event and
 //     completes its requests; latency is arrival -> completion observed.
 //   * Results do not depend on batching: the batched kernel computes every
@@ -50,6 +51,9 @@ struct ReqRec {
     int status = TC_OK;
     bool done = false;
 };
+
+// rows from which a served request gets its own launch (launch mode 0)
+constexpr int kAloneRows = 32768;
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -120,7 +124,20 @@ void dispatcher(tc_server *S)
         lk.unlock();
         reqs.resize(take);
         for (int i = 0; i < take; i++) reqs[i] = sl.reqs[i]->r;
-        int rc = tc::batch_enqueue(reqs.data(), take, sl.host_desc, sl.dev_desc, sl.stream);
+        // large requests run as their own launches (their kernels dwarf a
+        // launch; the batched kernels pay a per-tile descriptor lookup), small
+        // ones share one batched launch group
+        int rc = TC_OK;
+        int nsmall = 0;
+        for (int i = 0; i < take && rc == TC_OK; i++) {
+            const tc_request &q = reqs[i];
+            tc_matrix_stats ms{};
+            const bool big = S->cfg.launch == 0 && tc_matrix_get_stats(q.A, &ms) == TC_OK && ms.M >= kAloneRows;
+            const bool alone = S->cfg.launch == 2 || big;
+            if (alone) rc = tc_spmv(q.A, q.X, q.ldx, q.dtype, q.Y, q.ldy, q.k, sl.stream);
+            else reqs[nsmall++] = q;
+        }
+        if (rc == TC_OK && nsmall > 0) rc = tc::batch_enqueue(reqs.data(), nsmall, sl.host_desc, sl.dev_desc, sl.stream);
         if (rc == TC_OK && hipEventRecord(sl.ev, sl.stream) != hipSuccess) rc = TC_ERR_HIP;
         lk.lock();
         if (rc != TC_OK)
@@ -249,7 +266,8 @@ extern "C" {
 
 int tc_server_create(const tc_server_config *cfg, tc_server **out)
 {
-    if (!cfg || !out || cfg->max_batch < 1 || cfg->streams < 1 || cfg->max_inflight < 1 || cfg->hbm_cap_bytes < 0)
+    if (!cfg || !out || cfg->max_batch < 1 || cfg->streams < 1 || cfg->max_inflight < 1 || cfg->hbm_cap_bytes < 0 ||
+        cfg->launch < 0 || cfg->launch > 2)
         return tc::fail(TC_ERR_INVALID, "tc_server_create: bad config");
     *out = nullptr;
     int ndev = 0;
