@@ -16,9 +16,10 @@ int fail(int status, const char *fmt, ...) __attribute__((format(printf, 2, 3)))
 void clear_error();
 
 // Runs fn(begin, end) over [0, n) split into contiguous chunks on up to
-// `max_threads` std::threads (0 = hardware concurrency, capped at 16).
+// `max_threads` std::threads (0 = hardware concurrency, capped at 16); below
+// `grain` items it runs on the calling thread.
 void parallel_for(long long n, const std::function<void(long long, long long)> &fn,
-                  int max_threads = 0);
+                  int max_threads = 0, long long grain = 4096);
 
 // Counter-based random numbers: value i of stream `stream` under `seed`.
 // Independent of evaluation order, so generators give identical output on

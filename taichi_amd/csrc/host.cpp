@@ -2,6 +2,11 @@
 // reader/writer of libtaichi_amd.so (host code, built with g++).
 #include "common.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
@@ -29,12 +34,13 @@ int fail(int status, const char *fmt, ...)
 
 void clear_error() { g_last_error.clear(); }
 
-void parallel_for(long long n, const std::function<void(long long, long long)> &fn, int max_threads)
+void parallel_for(long long n, const std::function<void(long long, long long)> &fn, int max_threads,
+                  long long grain)
 {
     int hw = (int)std::thread::hardware_concurrency();
     if (hw <= 0) hw = 1;
     int nt = max_threads > 0 ? max_threads : std::min(hw, 16);
-    if (n < 4096 || nt <= 1) {
+    if (n < grain || nt <= 1) {
         if (n > 0) fn(0, n);
         return;
     }
@@ -120,6 +126,47 @@ struct Cursor {
 };
 
 // Banner fields as mm_read_banner classifies them. Returns 0 or an MM_* code.
+// A text file in memory with a NUL after its last byte.
+struct MappedText {
+    const char *data = nullptr;
+    size_t size = 0;
+    void *map = nullptr;
+    size_t map_len = 0;
+    std::vector<char> buf;
+
+    bool open(const char *path)
+    {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return false;
+        struct stat st;
+        const long page = sysconf(_SC_PAGESIZE);
+        if (fstat(fd, &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0 && page > 0 && st.st_size % page != 0) {
+            void *m = mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+            if (m != MAP_FAILED) {
+                ::close(fd);
+                map = m;
+                map_len = (size_t)st.st_size;
+                data = (const char *)m;
+                size = map_len;
+                return true;
+            }
+        }
+        char tmp[1 << 16];
+        ssize_t got;
+        while ((got = ::read(fd, tmp, sizeof tmp)) > 0) buf.insert(buf.end(), tmp, tmp + got);
+        ::close(fd);
+        if (got < 0) return false;
+        size = buf.size();
+        buf.push_back('\0');
+        data = buf.data();
+        return true;
+    }
+    ~MappedText()
+    {
+        if (map) munmap(map, map_len);
+    }
+};
+
 int parse_banner(const char *line, char tc[4])
 {
     char banner[64], mtx[64], crd[64], dtype[64], storage[64];
@@ -232,7 +279,7 @@ bool parse_lines_parallel(const char *p, const char *end, long long M, long long
                 q = le + 1;
             }
         }
-    }, T);
+    }, T, 1);
     long long total = 0;
     for (const Part &pt : parts) {
         if (!pt.ok) return false;
@@ -241,16 +288,20 @@ bool parse_lines_parallel(const char *p, const char *end, long long M, long long
     if (total < NZ) return false;
     ri.resize((size_t)NZ);
     ci.resize((size_t)NZ);
-    vi.assign((size_t)NZ, 1.0f);
-    long long o = 0;
-    for (const Part &pt : parts) {
-        const long long take = std::min<long long>((long long)pt.r.size(), NZ - o);
-        if (take <= 0) break;
-        std::copy(pt.r.begin(), pt.r.begin() + take, ri.begin() + o);
-        std::copy(pt.c.begin(), pt.c.begin() + take, ci.begin() + o);
-        if (kind != 'P') std::copy(pt.v.begin(), pt.v.begin() + take, vi.begin() + o);
-        o += take;
-    }
+    vi.resize((size_t)NZ);
+    std::vector<long long> off((size_t)T + 1, 0);
+    for (int t = 0; t < T; t++) off[t + 1] = off[t] + (long long)parts[t].r.size();
+    tc::parallel_for(T, [&](long long tb, long long te) {  // join in file order, first NZ entries
+        for (long long t = tb; t < te; t++) {
+            const Part &pt = parts[t];
+            const long long o = off[t], take = std::min<long long>((long long)pt.r.size(), NZ - o);
+            if (take <= 0) continue;
+            std::copy(pt.r.begin(), pt.r.begin() + take, ri.begin() + o);
+            std::copy(pt.c.begin(), pt.c.begin() + take, ci.begin() + o);
+            if (kind != 'P') std::copy(pt.v.begin(), pt.v.begin() + take, vi.begin() + o);
+            else std::fill(vi.begin() + o, vi.begin() + o + take, 1.0f);
+        }
+    }, T, 1);
     return true;
 }
 
@@ -261,17 +312,13 @@ extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA
 {
     if (!path || !m || !n || !nnzA || !rowptr || !colidx || !val)
         return tc::fail(TC_ERR_INVALID, "tc_read_mtx: null argument");
-    FILE *f = fopen(path, "rb");
-    if (!f) return tc::fail(-1, "cannot open %s", path);
-    std::vector<char> buf;
-    {
-        char tmp[1 << 16];
-        size_t got;
-        while ((got = fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + got);
-        fclose(f);
-    }
-    buf.push_back('\0');
-    const char *p = buf.data(), *end = buf.data() + buf.size() - 1;
+    // The file, NUL-terminated for the text scanners. A regular file whose
+    // size is not a multiple of the page size is mapped read-only with its
+    // pages populated up front (the bytes after EOF in the last page read as
+    // 0); anything else is read into memory.
+    MappedText text;
+    if (!text.open(path)) return tc::fail(-1, "cannot open %s", path);
+    const char *p = text.data, *end = text.data + text.size;
     // banner: first line (fgets of at most 1024 chars, mmio.cpp:109)
     const char *nl = (const char *)memchr(p, '\n', end - p);
     std::string first(p, nl ? nl : end);
@@ -301,10 +348,12 @@ extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA
     std::vector<float> vi;
     std::vector<long long> cnt((size_t)M + 1, 0);
     const bool parallel = parse_lines_parallel(cur.p, end, M, N, NZ, tc[2], ri, ci, vi);
+    bool sorted = true;  // entries already in row order (file order = CSR order)
     if (parallel) {
         for (long long i = 0; i < NZ; i++) {
             cnt[ri[i]]++;
             if (isSym && ri[i] != ci[i]) cnt[ci[i]]++;
+            if (i > 0 && ri[i] < ri[i - 1]) sorted = false;
         }
     } else {
         ri.resize((size_t)NZ);
@@ -330,6 +379,7 @@ extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA
         ri[i] = (int)a;
         ci[i] = (int)b;
         vi[i] = (float)fv;
+        if (i > 0 && ri[i] < ri[i - 1]) sorted = false;
         cnt[a]++;
         if (isSym && a != b) cnt[b]++;
     }
@@ -347,6 +397,21 @@ extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA
     }
     rp[0] = 0;
     for (long long r = 0; r < M; r++) rp[r + 1] = rp[r] + (int)cnt[r];
+    if (sorted && !isSym) {
+        // the CSR is the entries in file order (the scatter below would
+        // place each entry at its own position)
+        tc::parallel_for(NZ, [&](long long b, long long e) {
+            memcpy(cc + b, ci.data() + b, (size_t)(e - b) * sizeof(int));
+            memcpy(vv + b, vi.data() + b, (size_t)(e - b) * sizeof(float));
+        });
+        *m = (int)M;
+        *n = (int)N;
+        *nnzA = (unsigned long)total;
+        *rowptr = rp;
+        *colidx = cc;
+        *val = vv;
+        return 0;
+    }
     std::vector<int> fillp(rp, rp + M);
     for (long long i = 0; i < NZ; i++) {
         int a = ri[i], b = ci[i];
