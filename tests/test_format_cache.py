@@ -59,6 +59,7 @@ def test_damaged_files_are_rejected(tmp_path):
         "extended": raw + b"\0\0\0\0",
         "magic": b"X" + raw[1:],
         "checksum": raw[:-1] + bytes([raw[-1] ^ 0x80]),
+        "last_byte": raw[:-9] + bytes([raw[-9] ^ 0x01]) + raw[-8:],     # tail of the checksummed bytes
     }
     for what, data in cases.items():
         p = tmp_path / f"{what}.tcfmt"
