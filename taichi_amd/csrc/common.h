@@ -15,6 +15,11 @@ namespace tc {
 int fail(int status, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
 
+// Status for an exception caught at a C entry point (called from a catch
+// handler): TC_ERR_NOMEM for std::bad_alloc, TC_ERR_INVALID otherwise, with
+// the function name in the thread's last error. No exception crosses the C ABI.
+int exception_status(const char *fn);
+
 // Runs fn(begin, end) over [0, n) split into contiguous chunks on up to
 // `max_threads` std::threads (0 = hardware concurrency, capped at 16); below
 // `grain` items it runs on the calling thread.

@@ -127,7 +127,7 @@ int *take_ints(Reader &r, long long n, bool *ok)
 }  // namespace
 
 extern "C" int tc_save_format(const char *path, int M, int N, const TaiChi_new *t)
-{
+try {
     if (!path || !t || M < 0 || N < 0 || (M > 0 && !t->csrRow)) return tc::fail(TC_ERR_INVALID, "tc_save_format: bad argument");
     const int S = t->dia_shapes, D = t->num_negs;
     const long long R = M > 0 ? t->csrRow[M] : 0;
@@ -154,10 +154,12 @@ extern "C" int tc_save_format(const char *path, int M, int N, const TaiChi_new *
     w.ok = w.ok && fwrite(&sum, 1, sizeof sum, f) == sizeof sum;
     const bool ok = w.ok && fclose(f) == 0;
     return ok ? TC_OK : tc::fail(-1, "write error on %s", path);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_load_format(const char *path, int *M, int *N, TaiChi_new *t)
-{
+try {
     if (!path || !M || !N || !t) return tc::fail(TC_ERR_INVALID, "tc_load_format: bad argument");
     // the file mapped read-only with its pages populated (a file that cannot
     // be mapped is read into memory)
@@ -246,4 +248,6 @@ extern "C" int tc_load_format(const char *path, int *M, int *N, TaiChi_new *t)
     *N = n;
     *t = out;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }

@@ -85,7 +85,7 @@ int build_from_candidates(int n, long long items, Gen gen, int **rowptr, int **c
 
 extern "C" int tc_gen_diagonals_random(int n, const int *offsets, int noffsets, long long ndraws,
                                        uint64_t seed, int **rowptr, int **col, long long *nnz)
-{
+try {
     if (n <= 0 || noffsets < 0 || (noffsets && !offsets) || ndraws < 0 || !rowptr || !col || !nnz)
         return tc::fail(TC_ERR_INVALID, "tc_gen_diagonals_random: bad argument");
     std::vector<int> offs(offsets, offsets + noffsets);
@@ -107,11 +107,13 @@ extern "C" int tc_gen_diagonals_random(int n, const int *offsets, int noffsets, 
         }
     };
     return build_from_candidates(n, (long long)n + ndraws, gen, rowptr, col, nnz);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_gen_rmat(int n, int scale, long long nedges, double a, double b, double c,
                            uint64_t seed, int **rowptr, int **col, long long *nnz)
-{
+try {
     if (n <= 0 || scale < 0 || scale > 31 || ((long long)1 << scale) < n || nedges < 0 ||
         a < 0 || b < 0 || c < 0 || a + b + c > 1.0 || !rowptr || !col || !nnz)
         return tc::fail(TC_ERR_INVALID, "tc_gen_rmat: bad argument");
@@ -147,11 +149,13 @@ extern "C" int tc_gen_rmat(int n, int scale, long long nedges, double a, double 
         }
     };
     return build_from_candidates(n, nedges, gen, rowptr, col, nnz);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_gen_stencil5(int gx, int gy, double extra_frac, uint64_t seed, int **rowptr,
                                int **col, long long *nnz)
-{
+try {
     if (gx <= 0 || gy <= 0 || (long long)gx * gy > INT32_MAX || extra_frac < 0 || !rowptr || !col || !nnz)
         return tc::fail(TC_ERR_INVALID, "tc_gen_stencil5: bad argument");
     const long long n = (long long)gx * gy;
@@ -177,11 +181,13 @@ extern "C" int tc_gen_stencil5(int gx, int gy, double extra_frac, uint64_t seed,
         }
     };
     return build_from_candidates((int)n, n + extra, gen, rowptr, col, nnz);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_gen_banded(int n, int w, double density, uint64_t seed, int **rowptr, int **col,
                              long long *nnz)
-{
+try {
     if (n <= 0 || w <= 0 || density < 0 || density > 1 || !rowptr || !col || !nnz)
         return tc::fail(TC_ERR_INVALID, "tc_gen_banded: bad argument");
     const int lo = -(w / 2);
@@ -197,10 +203,12 @@ extern "C" int tc_gen_banded(int n, int w, double density, uint64_t seed, int **
             }
     };
     return build_from_candidates(n, n, gen, rowptr, col, nnz);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_gen_vector(float *out, long long n, int kind, uint64_t seed)
-{
+try {
     if (!out || n < 0 || kind < 0 || kind > 2) return tc::fail(TC_ERR_INVALID, "tc_gen_vector: bad argument");
     const uint64_t key = tc::rand_key(seed, 5 + (uint64_t)kind);
     tc::parallel_for(n, [&](long long b, long long e) {
@@ -218,4 +226,6 @@ extern "C" int tc_gen_vector(float *out, long long n, int kind, uint64_t seed)
         }
     });
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }

@@ -8,6 +8,8 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <exception>
+#include <new>
 #include <cctype>
 #include <cerrno>
 #include <cstdio>
@@ -33,6 +35,19 @@ int fail(int status, const char *fmt, ...)
 }
 
 void clear_error() { g_last_error.clear(); }
+
+int exception_status(const char *fn)
+{
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return fail(TC_ERR_NOMEM, "%s: out of host memory", fn);
+    } catch (const std::exception &e) {
+        return fail(TC_ERR_INVALID, "%s: %s", fn, e.what());
+    } catch (...) {
+        return fail(TC_ERR_INVALID, "%s: unexpected error", fn);
+    }
+}
 
 void parallel_for(long long n, const std::function<void(long long, long long)> &fn, int max_threads,
                   long long grain)
@@ -309,7 +324,7 @@ bool parse_lines_parallel(const char *p, const char *end, long long M, long long
 
 extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA, int **rowptr,
                            int **colidx, float **val)
-{
+try {
     if (!path || !m || !n || !nnzA || !rowptr || !colidx || !val)
         return tc::fail(TC_ERR_INVALID, "tc_read_mtx: null argument");
     // The file, NUL-terminated for the text scanners. A regular file whose
@@ -431,10 +446,12 @@ extern "C" int tc_read_mtx(const char *path, int *m, int *n, unsigned long *nnzA
     *colidx = cc;
     *val = vv;
     return 0;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_write_mtx(const char *path, int M, int N, const int *rowptr, const int *col)
-{
+try {
     if (!path || M < 0 || N < 0 || !rowptr || (rowptr[M] > 0 && !col))
         return tc::fail(TC_ERR_INVALID, "tc_write_mtx: bad argument");
     FILE *f = fopen(path, "w");
@@ -445,4 +462,6 @@ extern "C" int tc_write_mtx(const char *path, int M, int N, const int *rowptr, c
     int bad = ferror(f);
     fclose(f);
     return bad ? tc::fail(-1, "write error on %s", path) : TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }

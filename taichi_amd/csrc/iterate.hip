@@ -224,16 +224,18 @@ static int pagerank_args(tc_matrix *A, const float *inv_deg, float alpha, int it
 
 extern "C" int tc_pagerank(tc_matrix *A, const float *inv_deg, float alpha, int iters, float *r, void *work,
                            void *stream)
-{
+try {
     int n = 0;
     int rc = pagerank_args(A, inv_deg, alpha, iters, r, work, &n);
     if (rc != TC_OK || n == 0) return rc;
     return enqueue_pagerank(A, n, inv_deg, alpha, iters, r, carve(work, n), (hipStream_t)stream);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_pagerank_capture(tc_matrix *A, const float *inv_deg, float alpha, int iters, float *r,
                                    void *work, void *stream, tc_graph **out)
-{
+try {
     int n = 0;
     int rc = pagerank_args(A, inv_deg, alpha, iters, r, work, &n);
     if (rc != TC_OK) return rc;
@@ -262,4 +264,6 @@ extern "C" int tc_pagerank_capture(tc_matrix *A, const float *inv_deg, float alp
     }
     *out = g;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }

@@ -305,7 +305,7 @@ std::vector<int> contours_flat(const uint8_t *img, int H, int W)
 }  // namespace tc
 
 extern "C" int tc_sdf_from_image(const unsigned char *img, int H, int W, int *pts, int cap, int *num_shapes)
-{
+try {
     if (!img || H <= 0 || W <= 0 || !num_shapes || cap < 0 || (cap > 0 && !pts))
         return tc::fail(TC_ERR_INVALID, "tc_sdf_from_image: bad argument");
     std::vector<int> v = tc::sdf_from_image(img, H, W);
@@ -314,17 +314,21 @@ extern "C" int tc_sdf_from_image(const unsigned char *img, int H, int W, int *pt
     if (k > cap) return tc::fail(TC_ERR_CAPACITY, "tc_sdf_from_image: %d shapes > cap %d", k, cap);
     if (k) memcpy(pts, v.data(), v.size() * sizeof(int));
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_morph_from_image(const unsigned char *img, int H, int W, unsigned char *thin)
-{
+try {
     if (!img || !thin || H <= 0 || W <= 0) return tc::fail(TC_ERR_INVALID, "tc_morph_from_image: bad argument");
     tc::morph_thin(img, H, W, thin);
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_contours(const unsigned char *img, int H, int W, int *out, int cap, int *needed)
-{
+try {
     if (!img || H <= 0 || W <= 0 || !needed || cap < 0 || (cap > 0 && !out))
         return tc::fail(TC_ERR_INVALID, "tc_contours: bad argument");
     std::vector<int> v = tc::contours_flat(img, H, W);
@@ -332,4 +336,6 @@ extern "C" int tc_contours(const unsigned char *img, int H, int W, int *out, int
     if ((int)v.size() > cap) return tc::fail(TC_ERR_CAPACITY, "tc_contours: need %zu ints", v.size());
     memcpy(out, v.data(), v.size() * sizeof(int));
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }

@@ -177,7 +177,7 @@ extern "C" size_t tc_morph_work_bytes(int H, int W)
 
 extern "C" int tc_morph_device(const unsigned char *d_img, int H, int W, unsigned char *d_thin, void *work,
                                void *stream)
-{
+try {
     if (!d_img || !d_thin || !work || H <= 0 || W <= 0 || (long long)(H + 2) * (W + 2) > (1LL << 30))
         return tc::fail(TC_ERR_INVALID, "tc_morph_device: bad argument");
     hipStream_t s = (hipStream_t)stream;
@@ -207,6 +207,8 @@ extern "C" int tc_morph_device(const unsigned char *d_img, int H, int W, unsigne
     HIPCHECK(hipStreamSynchronize(s));
     if (herr) return tc::fail(TC_ERR_HIP, "tc_morph_device: union-find did not settle");
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 // getSdf with the morphology on the device: same points as tc_sdf_from_image

@@ -258,7 +258,7 @@ void tc_cal_start_end(int M_sub, int N_sub, int width, int off, int n, int *star
 
 int tc_partition_from_image(int M, int N, int nnz, int *rowptr, int *col, float *val,
                             const unsigned char *img, TaiChi_new *out)
-{
+try {
     if (M < 0 || N < 0 || nnz < 0 || !rowptr || (nnz > 0 && !col) || !out)
         return tc::fail(TC_ERR_INVALID, "tc_partition_from_image: bad argument");
     if (rowptr[0] != 0 || rowptr[M] != nnz)
@@ -325,6 +325,8 @@ int tc_partition_from_image(int M, int N, int nnz, int *rowptr, int *col, float 
         for (int j = 0; j < nnz; j++)
             if (claimed[j] && !pre[j]) val[j] = FLT_MAX;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 TaiChi_new *tc_taichi_new_alloc(void) { return (TaiChi_new *)calloc(1, sizeof(TaiChi_new)); }

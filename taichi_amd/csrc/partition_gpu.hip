@@ -324,7 +324,7 @@ int partition_on_device(int M, int N, int nnz, int *rowptr, int *col, const int 
 
 extern "C" int tc_partition_device(int M, int N, int nnz, int *rowptr, int *col, const int *d_rowptr,
                                    const int *d_col, TaiChi_new *out, void *stream)
-{
+try {
     if (M < 0 || N < 0 || nnz < 0 || !rowptr || (nnz > 0 && !col) || !out || (M > 0 && !d_rowptr) ||
         (nnz > 0 && !d_col))
         return tc::fail(TC_ERR_INVALID, "tc_partition_device: bad argument");
@@ -342,4 +342,6 @@ extern "C" int tc_partition_device(int M, int N, int nnz, int *rowptr, int *col,
         if (rc != TC_OK) return rc;
     }
     return partition_on_device(M, N, nnz, rowptr, col, d_rowptr, d_col, pts, out, s);
+} catch (...) {
+    return tc::exception_status(__func__);
 }

@@ -265,7 +265,7 @@ void fill_result(const std::vector<double> &lat_us, double secs, long long n_bat
 extern "C" {
 
 int tc_server_create(const tc_server_config *cfg, tc_server **out)
-{
+try {
     if (!cfg || !out || cfg->max_batch < 1 || cfg->streams < 1 || cfg->max_inflight < 1 || cfg->hbm_cap_bytes < 0 ||
         cfg->launch < 0 || cfg->launch > 2)
         return tc::fail(TC_ERR_INVALID, "tc_server_create: bad config");
@@ -290,6 +290,8 @@ int tc_server_create(const tc_server_config *cfg, tc_server **out)
     S->comp = std::thread(completer, S);
     *out = S;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_server_destroy(tc_server *S)
@@ -317,7 +319,7 @@ int tc_server_destroy(tc_server *S)
 }
 
 int tc_server_add_matrix(tc_server *S, int M, int N, const TaiChi_new *fmt, long long nnz, int *id)
-{
+try {
     if (!S || !fmt || !id) return tc::fail(TC_ERR_INVALID, "tc_server_add_matrix: bad argument");
     long long bytes = 0;
     int rc = tc_matrix_footprint(M, N, fmt, 0, &bytes);
@@ -340,10 +342,12 @@ int tc_server_add_matrix(tc_server *S, int M, int N, const TaiChi_new *fmt, long
     S->mats[mid] = Mat{A, bytes, 0};
     *id = mid;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_server_remove_matrix(tc_server *S, int id)
-{
+try {
     if (!S) return tc::fail(TC_ERR_INVALID, "tc_server_remove_matrix: null server");
     tc_matrix *A = nullptr;
     {
@@ -357,26 +361,32 @@ int tc_server_remove_matrix(tc_server *S, int id)
         S->mats.erase(it);
     }
     return tc_matrix_destroy(A);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_server_submit(tc_server *S, int matrix_id, const void *X, int ldx, int dtype, float *Y, int ldy, int k,
                      unsigned long long *ticket)
-{
+try {
     if (!S) return tc::fail(TC_ERR_INVALID, "tc_server_submit: null server");
     return submit_at(S, matrix_id, X, ldx, dtype, Y, ldy, k, now_ns(), ticket);
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_server_wait(tc_server *S, unsigned long long ticket, double *latency_us)
-{
+try {
     if (!S) return tc::fail(TC_ERR_INVALID, "tc_server_wait: null server");
     int64_t ns = 0;
     int rc = wait_ticket(S, ticket, &ns);
     if (latency_us) *latency_us = (double)ns * 1e-3;
     return rc;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_server_get_stats(tc_server *S, tc_server_stats *out)
-{
+try {
     if (!S || !out) return tc::fail(TC_ERR_INVALID, "tc_server_get_stats: bad argument");
     std::lock_guard<std::mutex> g(S->mu);
     out->resident_bytes = S->resident;
@@ -387,10 +397,12 @@ int tc_server_get_stats(tc_server *S, tc_server_stats *out)
     out->batches = S->n_batches;
     out->mean_batch = S->n_batches ? (double)S->n_batched_reqs / (double)S->n_batches : 0.0;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_server_run_load(tc_server *S, const tc_load_spec *sp, tc_load_result *res)
-{
+try {
     if (!S || !sp || !res || !sp->items || sp->nitems < 1 || !sp->ybufs || sp->nybufs < 1 || sp->count < 1)
         return tc::fail(TC_ERR_INVALID, "tc_server_run_load: bad spec");
     const bool open = sp->rate > 0;
@@ -445,10 +457,12 @@ int tc_server_run_load(tc_server *S, const tc_load_spec *sp, tc_load_result *res
     }
     fill_result(lat_us, secs, b1 - b0, r1 - r0, res);
     return rc;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_server_run_plain(tc_server *S, const tc_load_spec *sp, tc_load_result *res)
-{
+try {
     if (!S || !sp || !res || !sp->items || sp->nitems < 1 || !sp->ybufs || sp->nybufs < 1 || sp->count < 1)
         return tc::fail(TC_ERR_INVALID, "tc_server_run_plain: bad spec");
     hipStream_t s;
@@ -479,6 +493,8 @@ int tc_server_run_plain(tc_server *S, const tc_load_spec *sp, tc_load_result *re
     (void)hipStreamDestroy(s);
     fill_result(lat_us, secs, (long long)lat_us.size(), (long long)lat_us.size(), res);
     return rc;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 }  // extern "C"

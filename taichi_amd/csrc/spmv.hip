@@ -122,7 +122,7 @@ int tc_device_count(void)
 
 int tc_matrix_create(int M, int N, const TaiChi_new *T, long long nnz, int long_threshold, void *stream,
                      tc_matrix **out)
-{
+try {
     if (!T || !out || M < 0 || N < 0) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: bad argument");
     *out = nullptr;
     if (M > 0 && !T->csrRow) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: null csrRow");
@@ -316,10 +316,12 @@ int tc_matrix_create(int M, int N, const TaiChi_new *T, long long nnz, int long_
     st.device_bytes = (long long)total;
     *out = A;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_matrix_footprint(int M, int N, const TaiChi_new *T, int long_threshold, long long *bytes)
-{
+try {
     if (!T || !bytes || M < 0 || N < 0 || (M > 0 && !T->csrRow))
         return tc::fail(TC_ERR_INVALID, "tc_matrix_footprint: bad argument");
     const int LT = long_threshold > 0 ? std::min(long_threshold, 1023) : tc::kDefaultLongRow;
@@ -343,6 +345,8 @@ int tc_matrix_footprint(int M, int N, const TaiChi_new *T, int long_threshold, l
     for (long long b : sizes) total += (b + 255) & ~255ll;
     *bytes = total;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_matrix_destroy(tc_matrix *A)
@@ -354,10 +358,12 @@ int tc_matrix_destroy(tc_matrix *A)
 }
 
 int tc_matrix_get_stats(const tc_matrix *A, tc_matrix_stats *out)
-{
+try {
     if (!A || !out) return tc::fail(TC_ERR_INVALID, "tc_matrix_get_stats: null argument");
     *out = A->st;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 long long tc_spmv_bytes(const tc_matrix *A, int k, int dtype)
@@ -367,7 +373,7 @@ long long tc_spmv_bytes(const tc_matrix *A, int k, int dtype)
 }
 
 int tc_spmv(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int k, void *stream)
-{
+try {
     int rc = check_spmv_args(A, X, ldx, dtype, Y, ldy, k);
     if (rc != TC_OK) return rc;
     hipStream_t s = (hipStream_t)stream;
@@ -394,10 +400,12 @@ int tc_spmv(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, 
         c0 += K;
     }
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg)
-{
+try {
     if (!A || !cfg || k < 1) return tc::fail(TC_ERR_INVALID, "tc_matrix_set_config: bad argument");
     const int K = pick_k(k);
     tc_config c = *cfg;
@@ -412,18 +420,22 @@ int tc_matrix_set_config(tc_matrix *A, int k, int dtype, const tc_config *cfg)
     std::lock_guard<std::mutex> g(A->mu);
     A->cfg[{K, dtype}] = c;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_matrix_get_config(const tc_matrix *A, int k, int dtype, tc_config *cfg)
-{
+try {
     if (!A || !cfg || k < 1) return tc::fail(TC_ERR_INVALID, "tc_matrix_get_config: bad argument");
     *cfg = get_cfg(const_cast<tc_matrix *>(A), pick_k(k), dtype);
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int k, void *stream,
                 tc_config *best, float *best_us)
-{
+try {
     int rc = check_spmv_args(A, X, ldx, dtype, Y, ldy, k);
     if (rc != TC_OK) return rc;
     const int K = pick_k(k);
@@ -482,10 +494,12 @@ int tc_autotune(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int l
     if (best) *best = bc;
     if (best_us) *best_us = bt;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_spmv_batch(const tc_request *reqs, int n, void *stream)
-{
+try {
     if (n < 0 || (n > 0 && !reqs)) return tc::fail(TC_ERR_INVALID, "tc_spmv_batch: bad argument");
     if (n == 0) return TC_OK;
     hipStream_t s = (hipStream_t)stream;
@@ -497,11 +511,13 @@ int tc_spmv_batch(const tc_request *reqs, int n, void *stream)
     // the host descriptor array must outlive the async copy
     HIPCHECK(hipStreamSynchronize(s));
     return rc;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_graph_capture(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy, int k, int iters,
                      void *stream, tc_graph **out)
-{
+try {
     int rc = check_spmv_args(A, X, ldx, dtype, Y, ldy, k);
     if (rc != TC_OK) return rc;
     if (!out || iters < 1) return tc::fail(TC_ERR_INVALID, "tc_graph_capture: bad argument");
@@ -528,13 +544,17 @@ int tc_graph_capture(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, 
     }
     *out = g;
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_graph_launch(tc_graph *g, void *stream)
-{
+try {
     if (!g) return tc::fail(TC_ERR_INVALID, "tc_graph_launch: null graph");
     HIPCHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 int tc_graph_destroy(tc_graph *g)
@@ -548,7 +568,7 @@ int tc_graph_destroy(tc_graph *g)
 
 int tc_taichi_spmv(int M, int N, const TaiChi_new *T, const float *x_host, float *y_host, int warmup, int runs,
                    float *transfer_ms, float *spmv_ms)
-{
+try {
     if (!T || (N > 0 && !x_host) || (M > 0 && !y_host) || warmup < 0 || runs < 0)
         return tc::fail(TC_ERR_INVALID, "tc_taichi_spmv: bad argument");
     int ndev = 0;
@@ -598,6 +618,8 @@ int tc_taichi_spmv(int M, int N, const TaiChi_new *T, const float *x_host, float
 #undef STEP
     cleanup();
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 }  // extern "C"

@@ -95,7 +95,7 @@ extern "C" int tc_thumb_dims(int M, int N, int *H, int *W, int *pM, int *pN)
 }
 
 extern "C" int tc_thumbnail(int M, int N, const int *d_rowptr, const int *d_col, unsigned char *d_img, void *stream)
-{
+try {
     int H, W, pM, pN;
     int rc = tc_thumb_dims(M, N, &H, &W, &pM, &pN);
     if (rc != 0) return tc::fail(TC_ERR_INVALID, "tc_thumbnail: no thumbnail for M=%d N=%d (code %d)", M, N, rc);
@@ -120,11 +120,13 @@ extern "C" int tc_thumbnail(int M, int N, const int *d_rowptr, const int *d_col,
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipFreeAsync(bits, s));
     return TC_OK;
+} catch (...) {
+    return tc::exception_status(__func__);
 }
 
 extern "C" int tc_partition_shapes(const char *path, int M, int N, int nnz, int *rowptr, int *col, float *val,
                                    TaiChi_new *out)
-{
+try {
     (void)path;
     if (M < 0 || N < 0 || nnz < 0 || !rowptr || (nnz && !col) || !out)
         return tc::fail(TC_ERR_INVALID, "tc_partition_shapes: bad argument");
@@ -134,25 +136,32 @@ extern "C" int tc_partition_shapes(const char *path, int M, int N, int nnz, int 
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             return tc::fail(TC_ERR_NODEVICE, "tc_partition_shapes: no HIP device for the thumbnail kernel");
-        int *drow = nullptr, *dcol = nullptr;
-        unsigned char *dimg = nullptr;
+        struct Buffers {  // freed on every return path
+            int *drow = nullptr, *dcol = nullptr;
+            unsigned char *dimg = nullptr;
+            ~Buffers()
+            {
+                (void)hipFree(drow);
+                (void)hipFree(dcol);
+                (void)hipFree(dimg);
+            }
+        } b;
         img.resize((size_t)H * W);
-        HIPCHECK(hipMalloc((void **)&drow, ((size_t)M + 1) * sizeof(int)));
-        HIPCHECK(hipMalloc((void **)&dcol, (size_t)(nnz ? nnz : 1) * sizeof(int)));
-        HIPCHECK(hipMalloc((void **)&dimg, img.size()));
-        HIPCHECK(hipMemcpy(drow, rowptr, ((size_t)M + 1) * sizeof(int), hipMemcpyHostToDevice));
-        if (nnz) HIPCHECK(hipMemcpy(dcol, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHECK(hipMalloc((void **)&b.drow, ((size_t)M + 1) * sizeof(int)));
+        HIPCHECK(hipMalloc((void **)&b.dcol, (size_t)(nnz ? nnz : 1) * sizeof(int)));
+        HIPCHECK(hipMalloc((void **)&b.dimg, img.size()));
+        HIPCHECK(hipMemcpy(b.drow, rowptr, ((size_t)M + 1) * sizeof(int), hipMemcpyHostToDevice));
+        if (nnz) HIPCHECK(hipMemcpy(b.dcol, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
         int rc = TC_ERR_UNSUPPORTED;
-        if (!val) rc = tc_partition_device(M, N, nnz, rowptr, col, drow, dcol, out, nullptr);
+        if (!val) rc = tc_partition_device(M, N, nnz, rowptr, col, b.drow, b.dcol, out, nullptr);
         const bool host_build = rc == TC_ERR_UNSUPPORTED;  // val given, or duplicates on a dense diagonal
         if (host_build) {
-            rc = tc_thumbnail(M, N, drow, dcol, dimg, nullptr);
-            if (rc == TC_OK) HIPCHECK(hipMemcpy(img.data(), dimg, img.size(), hipMemcpyDeviceToHost));
+            rc = tc_thumbnail(M, N, b.drow, b.dcol, b.dimg, nullptr);
+            if (rc == TC_OK) HIPCHECK(hipMemcpy(img.data(), b.dimg, img.size(), hipMemcpyDeviceToHost));
         }
-        (void)hipFree(drow);
-        (void)hipFree(dcol);
-        (void)hipFree(dimg);
         if (!host_build || rc != TC_OK) return rc;
     }
     return tc_partition_from_image(M, N, nnz, rowptr, col, val, img.empty() ? nullptr : img.data(), out);
+} catch (...) {
+    return tc::exception_status(__func__);
 }

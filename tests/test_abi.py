@@ -97,3 +97,16 @@ def test_library_shares_torchs_hip_runtime():
     assert out.returncode == 0, out.stderr
     maps = ast.literal_eval(out.stdout.strip().splitlines()[-1])
     assert len(maps) == 1, maps
+
+
+def test_allocation_failure_is_a_status_not_a_crash():
+    """An exception inside a C entry point (here the morphology's buffers for a
+    2^30 x 2^30 image) comes back as TC_ERR_NOMEM, never out of the C ABI."""
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import numpy as np, taichi_amd as T\n"
+            "L = T.lib(); b = np.zeros(1, np.uint8)\n"
+            "print(L.tc_morph_from_image(b, 1 << 30, 1 << 30, b), L.tc_last_error().decode())" % ROOT)
+    out = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=300, cwd="/tmp")
+    assert out.returncode == 0, out.stderr[-2000:]
+    rc, msg = out.stdout.strip().splitlines()[-1].split(" ", 1)
+    assert int(rc) == -103 and "tc_morph_from_image" in msg, out.stdout
