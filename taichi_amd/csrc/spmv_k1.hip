@@ -167,12 +167,12 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
 
     if (full) {
         if constexpr (Q == 4)
-            *(float4 *)(Y + r0) = make_float4((float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
-        else *(float2 *)(Y + r0) = make_float2((float)acc[0], (float)acc[1]);
+            st_nt4(Y + r0, (float)acc[0], (float)acc[1], (float)acc[2], (float)acc[3]);
+        else st_nt2(Y + r0, (float)acc[0], (float)acc[1]);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; i++)
-            if (r0 + i < A.M) Y[r0 + i] = (float)acc[i];
+            if (r0 + i < A.M) st_nt(&Y[r0 + i], (float)acc[i]);
     }
 }
 

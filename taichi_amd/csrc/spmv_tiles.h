@@ -45,6 +45,23 @@ constexpr int kWG = 256;
 constexpr int kWaves = kWG / 64;
 constexpr int kStageU = 8;  // loads in flight per lane when staging streams
 
+// Non-temporal stores of 1, 2 or 4 floats (y: written once, not read back by
+// the kernel; streaming it past L2 leaves the cache to the x lines gathers
+// reuse -- measured 5% faster on R-MAT 2^22)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_nt(float *p, float a) { __builtin_nontemporal_store(a, p); }
+__device__ __forceinline__ void st_nt2(float *p, float a, float b)
+{
+    const f32x2 v = {a, b};
+    __builtin_nontemporal_store(v, (f32x2 *)p);
+}
+__device__ __forceinline__ void st_nt4(float *p, float a, float b, float c, float d)
+{
+    const f32x4 v = {a, b, c, d};
+    __builtin_nontemporal_store(v, (f32x4 *)p);
+}
+
 // Offset (in floats) of a double array placed after capv floats of LDS
 __host__ __device__ constexpr int lsum_at(int capv) { return (capv + 1) & ~1; }
 
@@ -418,7 +435,7 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
     // 5. store
 #pragma unroll
     for (int p = 0; p < P; p++)
-        if (pv[p]) Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]] = (float)acc[p];
+        if (pv[p]) st_nt(&Y[(size_t)(u0 + prow[p]) * ldy + pcol[p]], (float)acc[p]);
 }
 
 // ---- single right-hand side: lane-strided row kernel ----
@@ -550,7 +567,7 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             if (!pv[j]) continue;
-            Y[(size_t)(u0 + prow[j]) * ldy] = (float)acc[j];
+            st_nt(&Y[(size_t)(u0 + prow[j]) * ldy], (float)acc[j]);
         }
     }
 }
@@ -779,9 +796,8 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 continue;
             }
             if constexpr (CPL == 4)
-                *(float4 *)yp = make_float4((float)acc[i * 4], (float)acc[i * 4 + 1], (float)acc[i * 4 + 2],
-                                            (float)acc[i * 4 + 3]);
-            else *(float2 *)yp = make_float2((float)acc[i * 2], (float)acc[i * 2 + 1]);
+                st_nt4(yp, (float)acc[i * 4], (float)acc[i * 4 + 1], (float)acc[i * 4 + 2], (float)acc[i * 4 + 3]);
+            else st_nt2(yp, (float)acc[i * 2], (float)acc[i * 2 + 1]);
         }
     }
 }
