@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define TC_ABI_VERSION 1
+#define TC_ABI_VERSION 2
 
 /* ---- status codes (the reader's -1..-4 are the reference's own) ---- */
 #define TC_OK 0
@@ -182,6 +182,11 @@ typedef struct tc_matrix_stats {
     long long remainder;    /* Z + P entries stored in cols */
     long long format_bytes; /* bytes of every device array of the format */
     long long device_bytes; /* bytes allocated on the device for this matrix */
+    long long holes;        /* Z entries (cancelled zero cells of dense diagonals)
+                               held as cleared bits of a shape's term-presence
+                               bitmap instead of in cols */
+    int bitmap_shapes;      /* shapes with a term-presence bitmap */
+    int reserved;
 } tc_matrix_stats;
 
 /* upload a TaiChi_new (host) into the device layout. nnz is the count used

@@ -125,136 +125,18 @@ int tc_matrix_create(int M, int N, const TaiChi_new *T, long long nnz, int long_
 try {
     if (!T || !out || M < 0 || N < 0) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: bad argument");
     *out = nullptr;
-    if (M > 0 && !T->csrRow) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: null csrRow");
-    const int LT = long_threshold > 0 ? std::min(long_threshold, 1023) : tc::kDefaultLongRow;
-    const int S = T->dia_shapes;
-    const int R = M > 0 ? T->csrRow[M] : 0;
-    if (S < 0 || R < 0 || (R > 0 && !T->csrCol)) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: bad format");
-    if (S > 0 && (!T->xStart || !T->xStop || !T->yStart || !T->nDiasPtr || (T->nDiasPtr[S] > 0 && !T->neg_offsets)))
-        return tc::fail(TC_ERR_INVALID, "tc_matrix_create: null shape arrays");
-    const bool encoded = T->dense_nnz > 0;
-    if (!encoded && N >= (1 << 30)) return tc::fail(TC_ERR_UNSUPPORTED, "tc_matrix_create: N >= 2^30");
-    const int D = S > 0 ? T->nDiasPtr[S] : 0;
-
-    // shapes: validated so that every column the kernel forms lies in [0, N)
-    std::vector<ShapeDev> shapes((size_t)S);
-    for (int s = 0; s < S; s++) {
-        ShapeDev &h = shapes[s];
-        h.xs = T->xStart[s];
-        h.xe = T->xStop[s];
-        h.ys = T->yStart[s];
-        h.ob = T->nDiasPtr[s];
-        h.oe = T->nDiasPtr[s + 1];
-        if (h.xs < 0 || h.xe > M || h.xs > h.xe || h.ys < 0 || (long long)h.ys + (h.xe - h.xs) > N || h.ob < 0 ||
-            h.oe < h.ob || h.oe > D)
-            return tc::fail(TC_ERR_INVALID, "tc_matrix_create: shape %d out of range", s);
-        h.omin = 0;
-        h.omax = 0;
-        if (h.oe > h.ob) {
-            h.omin = h.omax = T->neg_offsets[h.ob];
-            for (int n = h.ob; n < h.oe; n++) {
-                h.omin = std::min(h.omin, T->neg_offsets[n]);
-                h.omax = std::max(h.omax, T->neg_offsets[n]);
-            }
-        }
-        h.pad = 0;
-    }
-    // rows: lengths, long rows, group offsets
-    const int G = (M + 63) / 64;
-    std::vector<int> gbase((size_t)G + 1, 0), lrow, lptr;
-    std::vector<uint16_t> roff16((size_t)M);
-    long long short_total = 0;
-    int maxoff = 0;
-    for (int r = 0; r < M; r++) {
-        const int b = T->csrRow[r], e = T->csrRow[r + 1];
-        if (e < b) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: csrRow not monotone at %d", r);
-        if ((r & 63) == 0) gbase[r >> 6] = (int)short_total;
-        if (e - b > LT) lrow.push_back(r);
-        else short_total += e - b;
-        const int o = (int)(short_total - gbase[r >> 6]);
-        roff16[r] = (uint16_t)o;
-        maxoff = std::max(maxoff, o);
-    }
-    gbase[G] = (int)short_total;
-    const int L = (int)lrow.size();
-    const int ow = maxoff <= 255 ? 1 : 2;
-    std::vector<int> glong;
-    if (L > 0) {
-        glong.assign((size_t)G + 1, 0);
-        for (int r : lrow) glong[(r >> 6) + 1]++;
-        for (int g = 0; g < G; g++) glong[g + 1] += glong[g];
-        lptr.resize((size_t)L + 1);
-        lptr[0] = (int)short_total;
-        for (int l = 0; l < L; l++) lptr[l + 1] = lptr[l] + (T->csrRow[lrow[l] + 1] - T->csrRow[lrow[l]]);
-    }
-    // encoded cols: short rows in row order, then long rows; checked against N
-    std::vector<int> cols((size_t)R);
-    {
-        std::vector<long long> dst((size_t)M);
-        long long sp = 0, lp = short_total;
-        size_t li = 0;
-        for (int r = 0; r < M; r++) {
-            const int len = T->csrRow[r + 1] - T->csrRow[r];
-            if (li < lrow.size() && lrow[li] == r) {
-                dst[r] = lp;
-                lp += len;
-                li++;
-            } else {
-                dst[r] = sp;
-                sp += len;
-            }
-        }
-        int bad = 0;
-        std::mutex bm;
-        tc::parallel_for(M, [&](long long b, long long e) {
-            int lbad = 0;
-            for (long long r = b; r < e; r++) {
-                long long o = dst[r];
-                for (int j = T->csrRow[r]; j < T->csrRow[r + 1]; j++) {
-                    const int raw = T->csrCol[j];
-                    const int c = encoded ? raw >> 1 : raw;
-                    if (raw < 0 || c >= N) {
-                        lbad = 1;
-                        cols[o++] = 1;
-                        continue;
-                    }
-                    cols[o++] = encoded ? raw : ((raw << 1) | 1);
-                }
-            }
-            if (lbad) {
-                std::lock_guard<std::mutex> g(bm);
-                bad = 1;
-            }
-        });
-        if (bad) return tc::fail(TC_ERR_INVALID, "tc_matrix_create: column index out of range");
-    }
+    Layout P;
+    int rc = plan_layout(M, N, T, long_threshold, P);
+    if (rc != TC_OK) return rc;
 
     // one device allocation, 256-byte aligned sub-arrays
-    struct Part {
-        const void *src;
-        size_t bytes;
-        size_t off;
-    };
-    std::vector<uint8_t> roff8;
-    if (ow == 1) {
-        roff8.resize((size_t)M);
-        for (int r = 0; r < M; r++) roff8[r] = (uint8_t)roff16[r];
-    }
-    Part parts[] = {
-        {shapes.data(), shapes.size() * sizeof(ShapeDev), 0},
-        {S > 0 ? T->neg_offsets : nullptr, (size_t)D * sizeof(int), 0},
-        {gbase.data(), gbase.size() * sizeof(int), 0},
-        {ow == 1 ? (const void *)roff8.data() : (const void *)roff16.data(), (size_t)M * ow, 0},
-        {cols.data(), cols.size() * sizeof(int), 0},
-        {glong.data(), glong.size() * sizeof(int), 0},
-        {lrow.data(), lrow.size() * sizeof(int), 0},
-        {lptr.data(), lptr.size() * sizeof(int), 0},
-    };
+    std::vector<LayoutPart> parts = layout_parts(P, P.S > 0 ? T->neg_offsets : nullptr);
+    std::vector<size_t> off(parts.size());
     size_t total = 0, fmt_bytes = 0;
-    for (Part &p : parts) {
-        p.off = total;
-        total += (p.bytes + 255) & ~(size_t)255;
-        fmt_bytes += p.bytes;
+    for (size_t i = 0; i < parts.size(); i++) {
+        off[i] = total;
+        total += (parts[i].bytes + 255) & ~(size_t)255;
+        fmt_bytes += parts[i].bytes;
     }
     tc_matrix *A = new tc_matrix();
     const hipError_t de = hipGetDevice(&A->device);
@@ -269,9 +151,9 @@ try {
             delete A;
             return tc::fail(TC_ERR_NOMEM, "tc_matrix_create: hipMalloc(%zu): %s", total, hipGetErrorString(e));
         }
-        for (Part &p : parts)
-            if (p.bytes) {
-                e = hipMemcpyAsync((char *)A->mem + p.off, p.src, p.bytes, hipMemcpyHostToDevice, s);
+        for (size_t i = 0; i < parts.size(); i++)
+            if (parts[i].bytes) {
+                e = hipMemcpyAsync((char *)A->mem + off[i], parts[i].src, parts[i].bytes, hipMemcpyHostToDevice, s);
                 if (e != hipSuccess) {
                     (void)hipFree(A->mem);
                     delete A;
@@ -285,35 +167,40 @@ try {
             return tc::fail(TC_ERR_HIP, "tc_matrix_create: sync: %s", hipGetErrorString(se));
         }
     }
-    auto at = [&](int i) -> const void * { return parts[i].bytes ? (const char *)A->mem + parts[i].off : nullptr; };
+    auto at = [&](int i) -> const void * { return parts[i].bytes ? (const char *)A->mem + off[i] : nullptr; };
     A->mem_bytes = total;
     MatView &v = A->v;
     v.M = M;
     v.N = N;
-    v.S = S;
-    v.L = L;
-    v.ow = ow;
-    v.ngroups = G;
+    v.S = P.S;
+    v.L = P.L;
+    v.ow = P.ow;
+    v.ngroups = P.G;
     v.shapes = (const ShapeDev *)at(0);
     v.offs = (const int *)at(1);
-    if (S > 0) v.sh0 = shapes[0];
+    if (P.S > 0) v.sh0 = P.shapes[0];
     v.gbase = (const int *)at(2);
     v.roff = at(3);
     v.cols = (const int *)at(4);
     v.glong = (const int *)at(5);
     v.lrow = (const int *)at(6);
     v.lptr = (const int *)at(7);
+    v.pres = (const uint32_t *)at(8);
     tc_matrix_stats &st = A->st;
     st.M = M;
     st.N = N;
-    st.dia_shapes = S;
-    st.num_negs = D;
-    st.long_rows = L;
-    st.offset_bytes = ow;
+    st.dia_shapes = P.S;
+    st.num_negs = P.D;
+    st.long_rows = P.L;
+    st.offset_bytes = P.ow;
     st.nnz = nnz;
-    st.remainder = R;
+    st.remainder = P.R;
     st.format_bytes = (long long)fmt_bytes;
     st.device_bytes = (long long)total;
+    st.holes = P.holes;
+    int pm_shapes = 0;
+    for (const ShapeDev &h : P.shapes) pm_shapes += h.pm >= 0;
+    st.bitmap_shapes = pm_shapes;
     *out = A;
     return TC_OK;
 } catch (...) {
@@ -324,25 +211,11 @@ int tc_matrix_footprint(int M, int N, const TaiChi_new *T, int long_threshold, l
 try {
     if (!T || !bytes || M < 0 || N < 0 || (M > 0 && !T->csrRow))
         return tc::fail(TC_ERR_INVALID, "tc_matrix_footprint: bad argument");
-    const int LT = long_threshold > 0 ? std::min(long_threshold, 1023) : tc::kDefaultLongRow;
-    const int S = std::max(T->dia_shapes, 0);
-    const long long D = S > 0 ? T->nDiasPtr[S] : 0;
-    const long long R = M > 0 ? T->csrRow[M] : 0;
-    const long long G = (M + 63) / 64;
-    long long L = 0, short_total = 0, gstart = 0;
-    int maxoff = 0;
-    for (int r = 0; r < M; r++) {
-        const int len = T->csrRow[r + 1] - T->csrRow[r];
-        if ((r & 63) == 0) gstart = short_total;
-        if (len > LT) L++;
-        else short_total += len;
-        maxoff = std::max(maxoff, (int)(short_total - gstart));
-    }
-    const int ow = maxoff <= 255 ? 1 : 2;
-    const long long sizes[] = {S * (long long)sizeof(ShapeDev), 4 * D, 4 * (G + 1), (long long)M * ow, 4 * R,
-                               L > 0 ? 4 * (G + 1) : 0, 4 * L, L > 0 ? 4 * (L + 1) : 0};
+    Layout P;
+    int rc = plan_layout(M, N, T, long_threshold, P);
+    if (rc != TC_OK) return rc;
     long long total = 0;
-    for (long long b : sizes) total += (b + 255) & ~255ll;
+    for (const LayoutPart &p : layout_parts(P, nullptr)) total += ((long long)p.bytes + 255) & ~255ll;
     *bytes = total;
     return TC_OK;
 } catch (...) {

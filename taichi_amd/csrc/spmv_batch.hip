@@ -263,7 +263,7 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
         const tc_request &r = reqs[i];
         const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
         const MatView &v = r.A->v;
-        const bool fast = v.S == 1 && v.sh0.oe - v.sh0.ob == 1;
+        const bool fast = v.S == 1 && v.sh0.oe - v.sh0.ob == 1 && v.sh0.pm < 0;
         chunks(r, [&](int ki, int K, int c0, int kv) {
             const int cl = chunk_class(r, ki, K, c0, dt, kv);
             WorkDesc &w = D[fill[cl]];

@@ -112,4 +112,24 @@ bool wide_applies(int K, const void *X, int ldx, const float *Y, int ldy, int es
 int launch_wide(const MatView &A, const void *X, int ldx, float *Y, int ldy, int dtype, int K, const tc_config &c,
                 hipStream_t s, int kv = 0);
 
+// Host copy of a matrix's device layout (layout.hip)
+struct Layout {
+    int M = 0, N = 0, S = 0, D = 0, G = 0, L = 0, ow = 1;
+    long long R = 0;      // entries kept in cols
+    long long holes = 0;  // cancelled zero cells absorbed by presence bitmaps
+    std::vector<ShapeDev> shapes;
+    std::vector<uint32_t> pres;
+    std::vector<int> gbase, glong, lrow, lptr, cols;
+    std::vector<uint16_t> roff16;  // local offsets when ow == 2
+    std::vector<uint8_t> roff8;    // local offsets when ow == 1
+};
+struct LayoutPart {
+    const void *src;
+    size_t bytes;
+};
+int plan_layout(int M, int N, const TaiChi_new *T, int long_threshold, Layout &P);
+// device arrays in upload order: shapes, offs, gbase, roff, cols, glong,
+// lrow, lptr, pres (offs: the format's neg_offsets)
+std::vector<LayoutPart> layout_parts(const Layout &P, const int *offs);
+
 }  // namespace tcd

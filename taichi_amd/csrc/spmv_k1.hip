@@ -151,7 +151,7 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
         for (int i = 0; i < Q; i++) {
             const int lo = max(rs[i], e0), hi = min(re[i], e0 + ne);
             double a = acc[i];
-            for (int k = lo; k < hi; k++) accum(a, s_val[k - e0]);
+            sum_run<1>(a, s_val, lo - e0, hi - e0);
             acc[i] = a;
         }
         __syncthreads();
@@ -204,7 +204,7 @@ int launch_rows_t(const MatView &A, const void *X, float *Y, int capv, int grid,
     const int ntiles = (A.M + TRW - 1) / TRW;
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
-    const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1;
+    const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1 && A.sh0.pm < 0;
     const size_t lds = ((size_t)lsum_at(capv) + (A.L > 0 ? 2 * TRW : 0) + (fast ? TRW : 0)) * 4;
     if (lds > max_lds_bytes())
         return tc::fail(TC_ERR_UNSUPPORTED, "rows tiles of %d with %d LDS values need %zu B of LDS", TRW, capv, lds);
@@ -276,7 +276,7 @@ int launch_quad_t(const MatView &A, const void *X, float *Y, int capv, int grid,
 bool quad_applies(const MatView &A, const void *X, const float *Y, int esize, const tc_config &c, int capv)
 {
     if (c.flags & kQuadOff) return false;
-    if (!(A.S == 0 || (A.S == 1 && A.sh0.oe - A.sh0.ob == 1))) return false;
+    if (!(A.S == 0 || (A.S == 1 && A.sh0.oe - A.sh0.ob == 1 && A.sh0.pm < 0))) return false;
     const int Q = c.rows_per_tile == 512 ? 2 : c.rows_per_tile == 1024 ? 4 : 0;
     if (Q == 0 || capv % 4 != 0 || capv < kWG * Q) return false;
     if (A.L > 0 && capv < 2 * kWG * Q) return false;  // the long-row sums (doubles) reuse s_val

@@ -15,6 +15,14 @@
 //   roff[M]     u8 or u16: end of row i's short entries, relative to gbase[i/64]
 //   cols[R]     encoded entries: short rows in row order, then long rows
 //   glong[G+1], lrow[L], lptr[L+1]   rows longer than the long-row threshold
+//   pres[]      term-presence bitmaps of the shapes that have them (pm >= 0):
+//               for shape s, word w (w < ceil(D_s/32)) of its local row i
+//               is pres[pm + w*rows_s + i]; bit j says whether the term of
+//               offset ob + j is present. A shape whose dense diagonals have
+//               many zero cells gets one: the sign-0 entries that cancel those
+//               cells leave cols and their bits are cleared, so the kernel
+//               adds only the present terms (values stay the same: a zero
+//               cell added and subtracted is a zero cell not added).
 // Compared with the reference's int rowptr (4 bytes per row) the per-row
 // cost is 1 (or 2) bytes plus 4 bytes per 64 rows.
 //
@@ -38,6 +46,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace tcd {
 
@@ -65,8 +74,11 @@ __device__ __forceinline__ void st_nt4(float *p, float a, float b, float c, floa
 // Offset (in floats) of a double array placed after capv floats of LDS
 __host__ __device__ constexpr int lsum_at(int capv) { return (capv + 1) & ~1; }
 
+// One shape: rows [xs, xe), offsets offs[ob..oe) (spread omin..omax), term
+// of row r and offset o at column ys + (r - xs) + o; pm = word offset of its
+// presence bitmap in pres, or -1 when every term is present.
 struct alignas(16) ShapeDev {
-    int xs, xe, ys, ob, oe, omin, omax, pad;
+    int xs, xe, ys, ob, oe, omin, omax, pm;
 };
 
 // Device pointers and sizes of one resident matrix; passed by value.
@@ -83,6 +95,7 @@ struct MatView {
     const int *glong;
     const int *lrow;
     const int *lptr;
+    const uint32_t *pres;
 };
 
 struct bf16_t {
@@ -128,8 +141,29 @@ __device__ __forceinline__ ShapeDev load_shape(const MatView &A, int s)
     h.oe = cload(p, 4);
     h.omin = cload(p, 5);
     h.omax = cload(p, 6);
-    h.pad = 0;
+    h.pm = cload(p, 7);
     return h;
+}
+
+// Presence words of row r in shape sh (sh.pm >= 0): bit j of lo (hi) is set
+// when the term of offset ob + j (ob + 32 + j) is present. Shapes with a
+// bitmap have at most 64 offsets. Rows outside the shape read its first row
+// (callers skip them).
+__device__ __forceinline__ void pres_words(const MatView &A, const ShapeDev &sh, int r, uint32_t &lo, uint32_t &hi)
+{
+    const int rows = sh.xe - sh.xs;
+    const int i = min(max(r - sh.xs, 0), rows - 1);
+    lo = A.pres[sh.pm + i];
+    hi = sh.oe - sh.ob > 32 ? A.pres[sh.pm + rows + i] : 0u;
+}
+
+// v if bit j of w is set, else +0.0, without a branch: the signed one-bit
+// field is 0 or all ones. Adding +0.0 leaves an FP64 sum that started at +0.0
+// unchanged (it can never be -0.0), so skipping a term and adding this zero
+// give the same bits.
+__device__ __forceinline__ float keep_if(float v, uint32_t w, int j)
+{
+    return __int_as_float(__float_as_int(v) & __builtin_amdgcn_sbfe((int)w, j, 1));
 }
 
 // start of row i's short entries (i in [0, M]); row i ends at row_start(i+1)
@@ -208,6 +242,22 @@ __device__ __forceinline__ void block_units(int total, int *b, int *e)
     }
 }
 
+// acc += s[i * K] for i in [lo, hi), in order, with four LDS reads in flight
+// (a lane's serial sum of its row's staged entries)
+template <int K>
+__device__ __forceinline__ void sum_run(double &acc, const float *s, int lo, int hi)
+{
+    int i = lo;
+    for (; i + 4 <= hi; i += 4) {
+        const float a = s[i * K], b = s[(i + 1) * K], c = s[(i + 2) * K], d = s[(i + 3) * K];
+        accum(acc, a);
+        accum(acc, b);
+        accum(acc, c);
+        accum(acc, d);
+    }
+    for (; i < hi; i++) accum(acc, s[i * K]);
+}
+
 // Stage the signed x values of ne contiguous entries (K columns each) into
 // s_val[j*K + c]. The loads of each batch (U items per lane) are all issued
 // before the first gather waits on them; lanes past the end re-read a valid
@@ -247,8 +297,9 @@ __device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne
 // Diagonal terms of one shape for the tile's pairs, in canonical order
 // (offsets as stored), added to acc. The x window of a shape with two
 // or more offsets is staged in LDS once per tile (s_val, left free after).
-template <int K, typename XT, int P>
-__device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
+// PM: the shape has a presence bitmap; absent terms are skipped.
+template <int K, typename XT, int P, bool PM>
+__device__ __forceinline__ void diag_one_shape_t(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
                                                const ShapeDev &sh, int ra, int rb, const int (&prow)[P],
                                                const int (&pcol)[P], const bool (&pv)[P], double (&acc)[P],
                                                float *s_val, int capv)
@@ -290,15 +341,50 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
         base[p] = r + shift;
         inner = inner && act[p] && base[p] + sh.omin >= clo && base[p] + sh.omax < chi;
     }
+    uint32_t plo[P], phi[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        plo[p] = phi[p] = 0;
+        if (PM) pres_words(A, sh, u0 + prow[p], plo[p], phi[p]);
+    }
+    // with a bitmap, offsets run in segments of 32 (one presence word each)
+    const int nseg = PM ? (sh.oe - sh.ob + 31) >> 5 : 1;
     if (stage) {
         int li[P];
 #pragma unroll
         for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
         if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
-            for (int n = sh.ob; n < sh.oe; n++) {
-                const int o = cload(A.offs, n) * K;
+            // DU offsets per step: their scalar loads, then all DU * P LDS
+            // reads, then the adds (each pair in offset order)
+            constexpr int DU = 8;
+            for (int seg = 0; seg < nseg; seg++) {
+                const int nb = sh.ob + 32 * seg, ne = PM ? min(sh.oe, nb + 32) : sh.oe;
+                uint32_t w[P];
 #pragma unroll
-                for (int p = 0; p < P; p++) accum(acc[p], s_val[li[p] + o]);
+                for (int p = 0; p < P; p++) w[p] = seg ? phi[p] : plo[p];
+                int n = nb;
+                for (; n + DU <= ne; n += DU) {
+                    int o[DU];
+#pragma unroll
+                    for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * K;
+                    float v[DU][P];
+#pragma unroll
+                    for (int u = 0; u < DU; u++)
+#pragma unroll
+                        for (int p = 0; p < P; p++) v[u][p] = s_val[li[p] + o[u]];
+#pragma unroll
+                    for (int u = 0; u < DU; u++)
+#pragma unroll
+                        for (int p = 0; p < P; p++) accum(acc[p], PM ? keep_if(v[u][p], w[p], n - nb + u) : v[u][p]);
+                }
+                for (; n < ne; n++) {
+                    const int o = cload(A.offs, n) * K;
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        const float v = s_val[li[p] + o];
+                        accum(acc[p], PM ? keep_if(v, w[p], n - nb) : v);
+                    }
+                }
             }
         } else {
             for (int n = sh.ob; n < sh.oe; n++) {
@@ -306,7 +392,11 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
 #pragma unroll
                 for (int p = 0; p < P; p++) {
                     const int c = base[p] + o;
-                    if (act[p] && c >= clo && c < chi) accum(acc[p], s_val[li[p] + o * K]);
+                    if (act[p] && c >= clo && c < chi) {
+                        const float v = s_val[li[p] + o * K];
+                        const int j = n - sh.ob;
+                        accum(acc[p], PM ? keep_if(v, j < 32 ? plo[p] : phi[p], j & 31) : v);
+                    }
                 }
             }
         }
@@ -317,10 +407,25 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 const int c = base[p] + o;
-                if (act[p] && c >= clo && c < chi) accum(acc[p], load_x<XT>(X, (size_t)c * ldx + pcol[p]));
+                if (act[p] && c >= clo && c < chi) {
+                    const float v = load_x<XT>(X, (size_t)c * ldx + pcol[p]);
+                    const int j = n - sh.ob;
+                    accum(acc[p], PM ? keep_if(v, j < 32 ? plo[p] : phi[p], j & 31) : v);
+                }
             }
         }
     }
+}
+
+template <int K, typename XT, int P>
+__device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
+                                               const ShapeDev &sh, int ra, int rb, const int (&prow)[P],
+                                               const int (&pcol)[P], const bool (&pv)[P], double (&acc)[P],
+                                               float *s_val, int capv)
+{
+    if (sh.pm >= 0)
+        diag_one_shape_t<K, XT, P, true>(A, X, ldx, u0, s, sh, ra, rb, prow, pcol, pv, acc, s_val, capv);
+    else diag_one_shape_t<K, XT, P, false>(A, X, ldx, u0, s, sh, ra, rb, prow, pcol, pv, acc, s_val, capv);
 }
 
 template <int K, typename XT, int P>
@@ -423,7 +528,7 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
             const int lo = max(s_start[prow[p]] - ebase, e0);
             const int hi = min(s_start[prow[p] + 1] - ebase, e0 + ne);
             double a = acc[p];
-            for (int j = lo; j < hi; j++) accum(a, s_val[(j - e0) * K + pcol[p]]);
+            sum_run<K>(a, s_val + pcol[p], lo - e0, hi - e0);
             acc[p] = a;
         }
         __syncthreads();
@@ -525,15 +630,38 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
             re[j] = pv[j] ? gbv[j] - ebase + endv[j] : rs[j];
         }
 
+        // cv holds the column indices of the next batch (the first one was
+        // loaded with the row offsets). Outside FAST a batch loads the indices
+        // of the one after it before its own gathers, so the two are in flight
+        // together and a tile of many entries pays one memory trip a batch,
+        // not two.
         for (int e0 = 0; e0 < E; e0 += capv) {
             const int ne = min(capv, E - e0);
             for (int b = 0; b < ne; b += kWG * U) {
                 int vv[U];
+                if constexpr (FAST) {
+                    // one shape of one offset (R-MAT-like): a tile's entries are
+                    // rarely more than one batch; no prefetch
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    vv[u] = 1;
-                    if (e0 == 0 && b == 0) vv[u] = cv[u];
-                    else if (b + u * kWG < ne) vv[u] = A.cols[ebase + e0 + min(b + u * kWG + tid, ne - 1)];
+                    for (int u = 0; u < U; u++) {
+                        vv[u] = 1;
+                        if (e0 == 0 && b == 0) vv[u] = cv[u];
+                        else if (b + u * kWG < ne) vv[u] = A.cols[ebase + e0 + min(b + u * kWG + tid, ne - 1)];
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; u++) vv[u] = cv[u];
+                    int nb = b + kWG * U, e2 = e0, ne2 = ne;  // the next batch: later in this round, or the next round's first
+                    if (nb >= ne) {
+                        e2 = e0 + capv;
+                        nb = 0;
+                        ne2 = min(capv, E - e2);
+                    }
+                    if (e2 < E) {
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                            cv[u] = nb + u * kWG < ne2 ? A.cols[ebase + e2 + min(nb + u * kWG + tid, ne2 - 1)] : 1;
+                    }
                 }
                 float xv[U];
 #pragma unroll
@@ -556,7 +684,11 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
             for (int j = 0; j < RPT; j++) {
                 const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
                 double a = acc[j];
-                for (int k = lo; k < hi; k++) accum(a, s_val[k - e0]);
+                if constexpr (FAST) {  // rows of a few entries: the plain loop
+                    for (int k = lo; k < hi; k++) accum(a, s_val[k - e0]);
+                } else {
+                    sum_run<1>(a, s_val, lo - e0, hi - e0);
+                }
                 acc[j] = a;
             }
             __syncthreads();
@@ -679,81 +811,155 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
         // of one offset are issued together. A shape with two or more
         // offsets first stages the tile's x window (rows [w0, w1), K columns)
         // in LDS, so each x row segment is loaded once per tile, not once
-        // per offset.
+        // per offset. Shapes with a presence bitmap (PM) skip absent terms.
         for (int s = 0; s < A.S; s++) {
             const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
             if (u0 + TRW <= sh.xs || u0 >= sh.xe) continue;
-            const int side = sh.xe - sh.xs;
-            const int shift = sh.ys - sh.xs;
-            const int ra = max(u0, sh.xs), rb = min(min(u0 + TRW, sh.xe), A.M);
-            const int w0 = max(sh.ys, ra + shift + sh.omin);
-            const int w1 = min(sh.ys + side, rb - 1 + shift + sh.omax + 1);
-            if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
-                const int nw = (w1 - w0) * LPR;  // CPL-wide vectors of the window
-                for (int b = 0; b < nw; b += kWG * 4) {
-                    float tv[4][CPL];
+            auto shape_terms = [&](auto pmc) {
+                constexpr bool PM = decltype(pmc)::value;
+                const int side = sh.xe - sh.xs;
+                const int shift = sh.ys - sh.xs;
+                const int ra = max(u0, sh.xs), rb = min(min(u0 + TRW, sh.xe), A.M);
+                const int w0 = max(sh.ys, ra + shift + sh.omin);
+                const int w1 = min(sh.ys + side, rb - 1 + shift + sh.omax + 1);
+                uint32_t plo[RI], phi[RI];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const int v = min(b + u * kWG + tid, nw - 1);
-                        const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
-                        if constexpr (MASK)
-                            load_qm<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == A.N - 1,
-                                             tv[u]);
-                        else load_q<XT, CPL>(xp, tv[u]);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const int v = b + u * kWG + tid;
-                        if (v < nw) {
-                            float *d = smem + (v / LPR) * K + (v % LPR) * CPL;
-                            if constexpr (CPL == 4) *(float4 *)d = make_float4(tv[u][0], tv[u][1], tv[u][2], tv[u][3]);
-                            else *(float2 *)d = make_float2(tv[u][0], tv[u][1]);
-                        }
-                    }
+                for (int i = 0; i < RI; i++) {
+                    plo[i] = phi[i] = 0;
+                    if (PM) pres_words(A, sh, u0 + GPW * i + g, plo[i], phi[i]);
                 }
-                __syncthreads();
-                for (int n = sh.ob; n < sh.oe; n++) {
-                    const int o = cload(A.offs, n);
+                if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
+                    const int nw = (w1 - w0) * LPR;  // CPL-wide vectors of the window
+                    for (int b = 0; b < nw; b += kWG * 4) {
+                        float tv[4][CPL];
 #pragma unroll
-                    for (int i = 0; i < RI; i++) {
-                        const int r = u0 + GPW * i + g;
-                        const int c = r + shift + o;
-                        if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
-                            const float *sp = smem + (c - w0) * K + c0;
-                            if constexpr (CPL == 4) {
-                                const float4 q = *(const float4 *)sp;
-                                accum(acc[i * 4], q.x), accum(acc[i * 4 + 1], q.y);
-                                accum(acc[i * 4 + 2], q.z), accum(acc[i * 4 + 3], q.w);
-                            } else {
-                                const float2 q = *(const float2 *)sp;
-                                accum(acc[i * 2], q.x), accum(acc[i * 2 + 1], q.y);
+                        for (int u = 0; u < 4; u++) {
+                            const int v = min(b + u * kWG + tid, nw - 1);
+                            const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
+                            if constexpr (MASK)
+                                load_qm<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL),
+                                                 w0 + v / LPR == A.N - 1, tv[u]);
+                            else load_q<XT, CPL>(xp, tv[u]);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const int v = b + u * kWG + tid;
+                            if (v < nw) {
+                                float *d = smem + (v / LPR) * K + (v % LPR) * CPL;
+                                if constexpr (CPL == 4)
+                                    *(float4 *)d = make_float4(tv[u][0], tv[u][1], tv[u][2], tv[u][3]);
+                                else *(float2 *)d = make_float2(tv[u][0], tv[u][1]);
                             }
                         }
                     }
-                }
-                __syncthreads();
-                continue;
-            }
-            for (int n = sh.ob; n < sh.oe; n++) {
-                const int o = cload(A.offs, n);
-                float xv[RI][CPL];
-                bool in[RI];
+                    __syncthreads();
+                    // every row of the wavefront inside the shape with all its
+                    // columns: DU offsets per step, their scalar loads, then
+                    // the DU * RI LDS vectors, then the adds in offset order
+                    bool inner = true;
 #pragma unroll
-                for (int i = 0; i < RI; i++) {
-                    const int r = u0 + GPW * i + g;
-                    const int c = r - sh.xs + sh.ys + o;
-                    in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
-                    if (in[i]) {
-                        if constexpr (MASK) load_qm<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xv[i]);
-                        else load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                    for (int i = 0; i < RI; i++) {
+                        const int r = u0 + GPW * i + g;
+                        inner = inner && r < A.M && r >= sh.xs && r < sh.xe && r + shift + sh.omin >= sh.ys &&
+                                r + shift + sh.omax < sh.ys + side;
                     }
+                    if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+                        constexpr int DU = 4;
+                        const int nseg = PM ? (sh.oe - sh.ob + 31) >> 5 : 1;
+                        int base[RI];
+#pragma unroll
+                        for (int i = 0; i < RI; i++) base[i] = (u0 + GPW * i + g + shift - w0) * K + c0;
+                        for (int seg = 0; seg < nseg; seg++) {
+                            const int nb = sh.ob + 32 * seg, ne = PM ? min(sh.oe, nb + 32) : sh.oe;
+                            uint32_t w[RI];
+#pragma unroll
+                            for (int i = 0; i < RI; i++) w[i] = seg ? phi[i] : plo[i];
+                            for (int n = nb; n < ne; n += DU) {
+                                const int nu = min(DU, ne - n);  // uniform
+                                int o[DU];
+#pragma unroll
+                                for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + min(u, nu - 1)) * K;
+                                float q[DU][RI][CPL];
+#pragma unroll
+                                for (int u = 0; u < DU; u++)
+#pragma unroll
+                                    for (int i = 0; i < RI; i++) {
+                                        const float *sp = smem + base[i] + o[u];
+                                        if constexpr (CPL == 4) {
+                                            const float4 t = *(const float4 *)sp;
+                                            q[u][i][0] = t.x, q[u][i][1] = t.y, q[u][i][2] = t.z, q[u][i][3] = t.w;
+                                        } else {
+                                            const float2 t = *(const float2 *)sp;
+                                            q[u][i][0] = t.x, q[u][i][1] = t.y;
+                                        }
+                                    }
+#pragma unroll
+                                for (int u = 0; u < DU; u++) {
+                                    if (u >= nu) break;
+#pragma unroll
+                                    for (int i = 0; i < RI; i++)
+#pragma unroll
+                                        for (int m = 0; m < CPL; m++)
+                                            accum(acc[i * CPL + m],
+                                                  PM ? keep_if(q[u][i][m], w[i], n - nb + u) : q[u][i][m]);
+                                }
+                            }
+                        }
+                        __syncthreads();
+                        return;
+                    }
+                    for (int n = sh.ob; n < sh.oe; n++) {
+                        const int o = cload(A.offs, n);
+#pragma unroll
+                        for (int i = 0; i < RI; i++) {
+                            const int r = u0 + GPW * i + g;
+                            const int c = r + shift + o;
+                            if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
+                                const float *sp = smem + (c - w0) * K + c0;
+                                const int j = n - sh.ob;
+                                const uint32_t w = j < 32 ? plo[i] : phi[i];
+                                auto kp = [&](float v) { return PM ? keep_if(v, w, j & 31) : v; };
+                                if constexpr (CPL == 4) {
+                                    const float4 q = *(const float4 *)sp;
+                                    accum(acc[i * 4], kp(q.x)), accum(acc[i * 4 + 1], kp(q.y));
+                                    accum(acc[i * 4 + 2], kp(q.z)), accum(acc[i * 4 + 3], kp(q.w));
+                                } else {
+                                    const float2 q = *(const float2 *)sp;
+                                    accum(acc[i * 2], kp(q.x)), accum(acc[i * 2 + 1], kp(q.y));
+                                }
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    return;
                 }
+                for (int n = sh.ob; n < sh.oe; n++) {
+                    const int o = cload(A.offs, n);
+                    float xv[RI][CPL];
+                    bool in[RI];
 #pragma unroll
-                for (int i = 0; i < RI; i++)
-                    if (in[i])
+                    for (int i = 0; i < RI; i++) {
+                        const int r = u0 + GPW * i + g;
+                        const int c = r - sh.xs + sh.ys + o;
+                        in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
+                        if (in[i]) {
+                            if constexpr (MASK) load_qm<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xv[i]);
+                            else load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                        }
+                    }
 #pragma unroll
-                        for (int m = 0; m < CPL; m++) accum(acc[i * CPL + m], xv[i][m]);
-            }
+                    for (int i = 0; i < RI; i++)
+                        if (in[i]) {
+                            const int j = n - sh.ob;
+                            const uint32_t w = j < 32 ? plo[i] : phi[i];
+#pragma unroll
+                            for (int m = 0; m < CPL; m++)
+                                accum(acc[i * CPL + m], PM ? keep_if(xv[i][m], w, j & 31) : xv[i][m]);
+                        }
+                }
+            };
+            if (sh.pm >= 0) shape_terms(std::true_type{});
+            else shape_terms(std::false_type{});
         }
 
         // remainder entries, U in flight, added in stored order
