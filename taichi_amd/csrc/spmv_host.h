@@ -26,6 +26,7 @@ struct tc_matrix {
     void *mem = nullptr;
     size_t mem_bytes = 0;
     tc_matrix_stats st{};
+    long long diag_terms = 0;  // sum over shapes of offsets x rows
     std::mutex mu;
     std::map<std::pair<int, int>, tc_config> cfg;
 };
