@@ -962,14 +962,22 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             else shape_terms(std::false_type{});
         }
 
-        // remainder entries, U in flight, added in stored order
+        // remainder entries, U in flight, added in stored order; the next U
+        // column indices load before this step's gathers, so a long row pays
+        // one memory trip per step, not two
 #pragma unroll
         for (int i = 0; i < RI; i++) {
+            int nx[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) nx[u] = rs[i] + u < re[i] ? A.cols[rs[i] + u] : 1;
             for (int j0 = rs[i]; j0 < re[i]; j0 += U) {
                 int v[U];
                 float xv[U][CPL];
 #pragma unroll
-                for (int u = 0; u < U; u++) v[u] = j0 + u < re[i] ? A.cols[j0 + u] : 1;
+                for (int u = 0; u < U; u++) v[u] = nx[u];
+                if (j0 + U < re[i])
+#pragma unroll
+                    for (int u = 0; u < U; u++) nx[u] = j0 + U + u < re[i] ? A.cols[j0 + U + u] : 1;
 #pragma unroll
                 for (int u = 0; u < U; u++)
                     if (j0 + u < re[i]) {
