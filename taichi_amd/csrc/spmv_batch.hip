@@ -83,15 +83,15 @@ __global__ __launch_bounds__(kWG) void batch_rows_kernel(const WorkDesc *__restr
         const XT *X = (const XT *)w.X;
         const bool lng = w.A.L > 0;
         if (w.A.ow == 1) {
-            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 1, true, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                            : rows_tile<XT, kBatchRPT, 1, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
-            else lng ? rows_tile<XT, kBatchRPT, 1, false, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                     : rows_tile<XT, kBatchRPT, 1, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 1, true, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, kBatchRPT, 1, true, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, kBatchRPT, 1, false, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, kBatchRPT, 1, false, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
         } else {
-            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 2, true, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                            : rows_tile<XT, kBatchRPT, 2, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
-            else lng ? rows_tile<XT, kBatchRPT, 2, false, true>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                     : rows_tile<XT, kBatchRPT, 2, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 2, true, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, kBatchRPT, 2, true, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, kBatchRPT, 2, false, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, kBatchRPT, 2, false, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
         }
     });
 }

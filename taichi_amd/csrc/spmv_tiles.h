@@ -356,7 +356,7 @@ __device__ __forceinline__ void diag_one_shape_t(const MatView &A, const XT *__r
         if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
             // DU offsets per step: their scalar loads, then all DU * P LDS
             // reads, then the adds (each pair in offset order)
-            constexpr int DU = 8;
+            constexpr int DU = 4;
             for (int seg = 0; seg < nseg; seg++) {
                 const int nb = sh.ob + 32 * seg, ne = PM ? min(sh.oe, nb + 32) : sh.oe;
                 uint32_t w[P];
@@ -553,7 +553,10 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
 // issued before the first use. Same canonical order as run_tile. Column c of
 // X is X[c * ldx], row r of Y is Y[r * ldy] (1 and 1 for a plain vector).
 // LDS: s_val[capv] | s_lsum[TRW] doubles (LONG) | s_win[TRW] (FAST).
-template <typename XT, int RPT, int OW, bool FAST, bool LONG>
+// PIPE (general path only): prefetch the next batch's column indices and sum
+// rows with four LDS reads in flight; the batched kernel, whose registers
+// serve every matrix kind in one launch, leaves it off.
+template <typename XT, int RPT, int OW, bool FAST, bool LONG, bool PIPE = true>
 __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
                                           int ldy, int t, float *smem, int capv)
 {
@@ -639,7 +642,7 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
             const int ne = min(capv, E - e0);
             for (int b = 0; b < ne; b += kWG * U) {
                 int vv[U];
-                if constexpr (FAST) {
+                if constexpr (FAST || !PIPE) {
                     // one shape of one offset (R-MAT-like): a tile's entries are
                     // rarely more than one batch; no prefetch
 #pragma unroll
@@ -684,7 +687,7 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
             for (int j = 0; j < RPT; j++) {
                 const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
                 double a = acc[j];
-                if constexpr (FAST) {  // rows of a few entries: the plain loop
+                if constexpr (FAST || !PIPE) {  // rows of a few entries: the plain loop
                     for (int k = lo; k < hi; k++) accum(a, s_val[k - e0]);
                 } else {
                     sum_run<1>(a, s_val, lo - e0, hi - e0);
