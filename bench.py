@@ -171,8 +171,15 @@ def bench_served(dev, requests=40000, rate=20000.0):
         f"built in {time.time() - t0:.1f}s")
     srv.run_load(items, ybufs, 8, count=2000, rate=0.0, concurrency=256, seed=1)  # warm-up
     poisson = srv.run_load(items, ybufs, 8, count=requests, rate=rate, seed=2)
+    # CLOCK_MONOTONIC windows of the closed loop and the plain loop, the clock
+    # a kernel trace is stamped with, to relate device busy time to each
+    torch.cuda.synchronize()
+    w0 = time.monotonic_ns()
     closed = srv.run_load(items, ybufs, 8, count=requests, rate=0.0, concurrency=256, seed=3)
+    w1 = time.monotonic_ns()
     plain = srv.run_plain(items, ybufs, 8, count=max(2000, requests // 10), seed=4)
+    w2 = time.monotonic_ns()
+    log(f"served windows (monotonic ns): closed {w0} {w1} plain {w1} {w2}")
     srv.close()
 
     def pick(r):

@@ -97,8 +97,8 @@ int launch_cfg(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ld
     if (K == 1 && ldx == 1 && ldy == 1 && quad_applies(A->v, X, Y, elem_size(dtype), c, capv)) {
         return launch_quad(A->v, X, Y, dtype, c, capv, s);
     }
-    if (K == 1 && ldx == 1 && ldy == 1 && ((uintptr_t)Y & 15) == 0) {
-        return launch_rows(A->v, X, Y, dtype, c, capv, s);
+    if (K == 1 && ((uintptr_t)Y & 15) == 0) {
+        return launch_rows(A->v, X, ldx, Y, ldy, dtype, c, capv, s);
     }
     if (K > 1 && wide_applies(K, X, ldx, Y, ldy, elem_size(dtype), c)) {
         return launch_wide(A->v, X, ldx, Y, ldy, dtype, K, c, s);

@@ -100,7 +100,9 @@ int check_spmv_args(const tc_matrix *A, const void *X, int ldx, int dtype, const
 // per-workgroup LDS limit of the current device (cached per device)
 size_t max_lds_bytes();
 
-int launch_rows(const MatView &A, const void *X, float *Y, int dtype, const tc_config &c, int capv, hipStream_t s);
+// X column: X[r * ldx], Y column: Y[r * ldy]
+int launch_rows(const MatView &A, const void *X, int ldx, float *Y, int ldy, int dtype, const tc_config &c, int capv,
+                hipStream_t s);
 // k = 1, quad rows: whether they apply, and the launch (spmv_k1.hip)
 bool quad_applies(const MatView &A, const void *X, const float *Y, int esize, const tc_config &c, int capv);
 int launch_quad(const MatView &A, const void *X, float *Y, int dtype, const tc_config &c, int capv, hipStream_t s);

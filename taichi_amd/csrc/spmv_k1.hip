@@ -5,14 +5,17 @@
 
 namespace tcd {
 
-template <typename XT, int RPT, int OW, bool FAST, bool LONG>
-__global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__restrict__ X, float *__restrict__ Y,
-                                                        int ntiles, int capv)
+// STR: x and y are one column of row-major X and Y (leading dimensions ldx,
+// ldy), as a served request's single column is; otherwise plain vectors.
+template <typename XT, int RPT, int OW, bool FAST, bool LONG, bool STR>
+__global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__restrict__ X, int ldx,
+                                                        float *__restrict__ Y, int ldy, int ntiles, int capv)
 {
     extern __shared__ float smem[];
     int tb, te;
     block_units(ntiles, &tb, &te);
-    for (int t = tb; t < te; t++) rows_tile<XT, RPT, OW, FAST, LONG>(A, X, 1, Y, 1, t, smem, capv);
+    for (int t = tb; t < te; t++)
+        rows_tile<XT, RPT, OW, FAST, LONG>(A, X, STR ? ldx : 1, Y, STR ? ldy : 1, t, smem, capv);
 }
 
 // ---- single right-hand side, quad rows: at most one shape with one offset ----
@@ -192,13 +195,18 @@ __global__ __launch_bounds__(kWG) void spmv_quad_kernel(MatView A, const XT *__r
 // ---- launchers ----
 
 template <typename XT, int RPT, int OW, bool FAST, bool LONG>
-void launch_rows_k(const MatView &A, const void *X, float *Y, int ntiles, int capv, int G, size_t lds, hipStream_t s)
+void launch_rows_k(const MatView &A, const void *X, int ldx, float *Y, int ldy, int ntiles, int capv, int G,
+                   size_t lds, hipStream_t s)
 {
-    spmv_rows_kernel<XT, RPT, OW, FAST, LONG><<<G, kWG, lds, s>>>(A, (const XT *)X, Y, ntiles, capv);
+    if (ldx == 1 && ldy == 1)
+        spmv_rows_kernel<XT, RPT, OW, FAST, LONG, false><<<G, kWG, lds, s>>>(A, (const XT *)X, 1, Y, 1, ntiles, capv);
+    else if constexpr (RPT <= 2)
+        spmv_rows_kernel<XT, RPT, OW, FAST, LONG, true><<<G, kWG, lds, s>>>(A, (const XT *)X, ldx, Y, ldy, ntiles,
+                                                                            capv);
 }
 
 template <typename XT, int RPT>
-int launch_rows_t(const MatView &A, const void *X, float *Y, int capv, int grid, hipStream_t s)
+int launch_rows_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int capv, int grid, hipStream_t s)
 {
     constexpr int TRW = kWG * RPT;
     const int ntiles = (A.M + TRW - 1) / TRW;
@@ -212,29 +220,36 @@ int launch_rows_t(const MatView &A, const void *X, float *Y, int capv, int grid,
     // specialised on offset width, the one-shape-one-offset case and long
     // rows, so the common case carries no registers for the general paths
     if (A.ow == 1) {
-        if (fast) lng ? launch_rows_k<XT, RPT, 1, true, true>(A, X, Y, ntiles, capv, G, lds, s)
-                      : launch_rows_k<XT, RPT, 1, true, false>(A, X, Y, ntiles, capv, G, lds, s);
-        else lng ? launch_rows_k<XT, RPT, 1, false, true>(A, X, Y, ntiles, capv, G, lds, s)
-                 : launch_rows_k<XT, RPT, 1, false, false>(A, X, Y, ntiles, capv, G, lds, s);
+        if (fast) lng ? launch_rows_k<XT, RPT, 1, true, true>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s)
+                      : launch_rows_k<XT, RPT, 1, true, false>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s);
+        else lng ? launch_rows_k<XT, RPT, 1, false, true>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s)
+                 : launch_rows_k<XT, RPT, 1, false, false>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s);
     } else {
-        if (fast) lng ? launch_rows_k<XT, RPT, 2, true, true>(A, X, Y, ntiles, capv, G, lds, s)
-                      : launch_rows_k<XT, RPT, 2, true, false>(A, X, Y, ntiles, capv, G, lds, s);
-        else lng ? launch_rows_k<XT, RPT, 2, false, true>(A, X, Y, ntiles, capv, G, lds, s)
-                 : launch_rows_k<XT, RPT, 2, false, false>(A, X, Y, ntiles, capv, G, lds, s);
+        if (fast) lng ? launch_rows_k<XT, RPT, 2, true, true>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s)
+                      : launch_rows_k<XT, RPT, 2, true, false>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s);
+        else lng ? launch_rows_k<XT, RPT, 2, false, true>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s)
+                 : launch_rows_k<XT, RPT, 2, false, false>(A, X, ldx, Y, ldy, ntiles, capv, G, lds, s);
     }
     HIPCHECK(hipGetLastError());
     return TC_OK;
 }
 
+// strided columns are compiled for tiles of 256 and 512 rows only; other
+// tile sizes run them as 512
 template <typename XT>
-int launch_rows_x(const MatView &A, const void *X, float *Y, const tc_config &c, int capv, hipStream_t s)
+int launch_rows_x(const MatView &A, const void *X, int ldx, float *Y, int ldy, const tc_config &c, int capv,
+                  hipStream_t s)
 {
+    const bool str = ldx != 1 || ldy != 1;
     switch (c.rows_per_tile) {
-    case 256: return launch_rows_t<XT, 1>(A, X, Y, capv, c.grid, s);
-    case 512: return launch_rows_t<XT, 2>(A, X, Y, capv, c.grid, s);
-    case 1024: return launch_rows_t<XT, 4>(A, X, Y, capv, c.grid, s);
-    case 4096: return launch_rows_t<XT, 16>(A, X, Y, capv, c.grid, s);
-    default: return launch_rows_t<XT, 8>(A, X, Y, capv, c.grid, s);
+    case 256: return launch_rows_t<XT, 1>(A, X, ldx, Y, ldy, capv, c.grid, s);
+    case 512: return launch_rows_t<XT, 2>(A, X, ldx, Y, ldy, capv, c.grid, s);
+    }
+    if (str) return launch_rows_t<XT, 2>(A, X, ldx, Y, ldy, capv, c.grid, s);
+    switch (c.rows_per_tile) {
+    case 1024: return launch_rows_t<XT, 4>(A, X, 1, Y, 1, capv, c.grid, s);
+    case 4096: return launch_rows_t<XT, 16>(A, X, 1, Y, 1, capv, c.grid, s);
+    default: return launch_rows_t<XT, 8>(A, X, 1, Y, 1, capv, c.grid, s);
     }
 }
 
@@ -290,11 +305,12 @@ int launch_quad_x(const MatView &A, const void *X, float *Y, const tc_config &c,
     return launch_quad_t<XT, 2>(A, X, Y, capv, c.grid, s);
 }
 
-int launch_rows(const MatView &A, const void *X, float *Y, int dtype, const tc_config &c, int capv, hipStream_t s)
+int launch_rows(const MatView &A, const void *X, int ldx, float *Y, int ldy, int dtype, const tc_config &c, int capv,
+                hipStream_t s)
 {
-    if (dtype == TC_BF16) return launch_rows_x<bf16_t>(A, X, Y, c, capv, s);
-    if (dtype == TC_F16) return launch_rows_x<f16_t>(A, X, Y, c, capv, s);
-    return launch_rows_x<float>(A, X, Y, c, capv, s);
+    if (dtype == TC_BF16) return launch_rows_x<bf16_t>(A, X, ldx, Y, ldy, c, capv, s);
+    if (dtype == TC_F16) return launch_rows_x<f16_t>(A, X, ldx, Y, ldy, c, capv, s);
+    return launch_rows_x<float>(A, X, ldx, Y, ldy, c, capv, s);
 }
 
 int launch_quad(const MatView &A, const void *X, float *Y, int dtype, const tc_config &c, int capv, hipStream_t s)

@@ -414,3 +414,38 @@ def test_masked_column_chunks(case, dev):
         for c in range(k):
             assert torch.equal(Ys[:, c], cols1[c]), (k, c)
             assert torch.equal(Yb[:, c], cols1[c]), (k, c)
+
+
+@pytest.mark.parametrize("kind", ["rmat", "stencil", "band"])
+def test_strided_single_column(kind, dev):
+    """k = 1 on one column of row-major X and Y (ldx = ldy = 8, as a served
+    request's column is) runs the one-row-per-lane kernel with strides and
+    gives the plain vector's result, bit for bit, for every tile size and x
+    type; integer x matches the CPU sum exactly."""
+    if kind == "rmat":
+        M, N, rp, col = rmat(15)
+    elif kind == "stencil":
+        M = N = 128 * 128
+        rp, col = T.gen_diagonals_random(M, [-128, -1, 0, 1, 128], M // 50, 3)
+    else:
+        M = N = 40000
+        rp, col = T.gen_banded(M, 40, 0.7, 5)
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    A = T.Matrix(fmt)
+    rng = np.random.default_rng(M)
+    xi = rng.integers(0, 8, N).astype(np.float32)
+    want = O.csr_spmv_f32(rp, col, xi)
+    for dt in (torch.float32, torch.bfloat16):
+        X8 = torch.zeros(N, 8, device=dev, dtype=dt)
+        X8[:, 3] = torch.from_numpy(xi).to(dev).to(dt)
+        xv = X8[:, 3].contiguous()
+        for rows in (256, 512, 1024):
+            A.set_config(1, T.BF16 if dt == torch.bfloat16 else T.F32, rows, 4096, 0, 2)
+            Y8 = torch.full((M, 8), float("nan"), device=dev)
+            A.spmv(X8[:, 3:4], Y8[:, 3:4])
+            yv = torch.full((M,), float("nan"), device=dev)
+            A.spmv(xv, yv)
+            torch.cuda.synchronize()
+            assert torch.equal(Y8[:, 3], yv), (dt, rows)
+            assert torch.isnan(Y8[:, :3]).all() and torch.isnan(Y8[:, 4:]).all(), (dt, rows)
+            assert np.array_equal(yv.cpu().numpy(), want), (dt, rows)
