@@ -38,13 +38,13 @@ int cu_count()
 // Wide bands also take two rows per lane group at k <= 8 (the window
 // staged once serves twice the rows): 57 diagonals of 923k rows, k = 8
 // measured 62 us against 72 us with one.
-tc_config default_config(int K, long long remainder, long long diag_terms, int M)
+tc_config default_config(int K, int esize, long long remainder, long long diag_terms, int M)
 {
     tc_config c{};
     const bool band = M > 0 && remainder + diag_terms > 24ll * M;
     const bool heavy = M > 0 && (remainder > 8ll * M || band);
     c.rows_per_tile = K == 1      ? (heavy ? 256 : 512)
-                      : wide_k(K) ? wide_rows(K, band && K <= 8 ? 2 : 1)
+                      : wide_k(K) ? wide_rows(K, band && K <= 8 ? 2 : 1, esize)
                                   : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
@@ -52,9 +52,9 @@ tc_config default_config(int K, long long remainder, long long diag_terms, int M
     return c;
 }
 
-tc_config default_config(const tc_matrix *A, int K)
+tc_config default_config(const tc_matrix *A, int K, int dtype)
 {
-    return default_config(K, A->st.remainder, A->diag_terms, A->v.M);
+    return default_config(K, elem_size(dtype), A->st.remainder, A->diag_terms, A->v.M);
 }
 
 size_t max_lds_bytes()
@@ -72,10 +72,10 @@ size_t max_lds_bytes()
     return v;
 }
 
-bool rows_per_tile_ok(int K, int r, int flags)
+bool rows_per_tile_ok(int K, int r, int flags, int esize)
 {
     if (K == 1) return r == 256 || r == 512 || r == 1024 || r == 2048 || r == 4096;
-    if (wide_k(K) && !(flags & kStagedK)) return r == wide_rows(K, 1) || r == wide_rows(K, 2);
+    if (wide_k(K) && !(flags & kStagedK)) return r == wide_rows(K, 1, esize) || r == wide_rows(K, 2, esize);
     for (int P : {1, 2, 4})
         if ((kWG * P) / K == r) return true;
     return false;
@@ -85,7 +85,7 @@ tc_config get_cfg(tc_matrix *A, int K, int dtype)
 {
     std::lock_guard<std::mutex> g(A->mu);
     auto it = A->cfg.find({K, dtype});
-    return it != A->cfg.end() ? it->second : default_config(A, K);
+    return it != A->cfg.end() ? it->second : default_config(A, K, dtype);
 }
 
 
@@ -295,10 +295,10 @@ try {
     if (!A || !cfg || k < 1) return tc::fail(TC_ERR_INVALID, "tc_matrix_set_config: bad argument");
     const int K = pick_k(k);
     tc_config c = *cfg;
-    const tc_config d = default_config(A, K);
+    const tc_config d = default_config(A, K, dtype);
     if (c.rows_per_tile <= 0) c.rows_per_tile = d.rows_per_tile;
     if (c.lds_values <= 0) c.lds_values = d.lds_values;
-    if (!rows_per_tile_ok(K, c.rows_per_tile, c.flags))
+    if (!rows_per_tile_ok(K, c.rows_per_tile, c.flags, elem_size(dtype)))
         return tc::fail(TC_ERR_INVALID, "rows_per_tile %d not available for k=%d flags=%d", c.rows_per_tile, K, c.flags);
     if (c.lds_values < 512 || c.lds_values > 8192)
         return tc::fail(TC_ERR_INVALID, "lds_values %d outside [512, 8192]", c.lds_values);
@@ -339,7 +339,7 @@ try {
             for (int f : {0, kQuadOff}) tf.push_back({r, f});
     } else {
         if (wide_k(K))
-            for (int RI : {1, 2}) tf.push_back({wide_rows(K, RI), 0});
+            for (int RI : {1, 2}) tf.push_back({wide_rows(K, RI, elem_size(dtype)), 0});
         for (int P : {1, 2, 4}) tf.push_back({(kWG * P) / K, kStagedK});
     }
     for (auto [rpt, flags] : tf)
@@ -351,7 +351,7 @@ try {
                 tc_config c{rpt, capv, grid, flags};
                 cands.push_back(c);
             }
-    tc_config bc = default_config(A, K);
+    tc_config bc = default_config(A, K, dtype);
     float bt = 1e30f;
     const int reps = 20;
     for (const tc_config &c : cands) {

@@ -149,10 +149,10 @@ size_t batch_desc_bytes(int max_requests)
 constexpr int kBatchClasses = 4 * 4 * 3;
 inline int batch_class(int kern, int ki, int dt) { return (kern * 4 + ki) * 3 + dt; }
 
-int batch_tile_rows(int kern, int K)
+int batch_tile_rows(int kern, int K, int esize)
 {
     if (kern == 0) return kWG * kBatchRPT;
-    if (kern == 1 || kern == 3) return wide_rows(K, kBatchRI);
+    if (kern == 1 || kern == 3) return wide_rows(K, kBatchRI, esize);
     return (kWG * kBatchP) / K;
 }
 
@@ -167,8 +167,8 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
         const size_t lds = ((size_t)lsum_at(kBatchCapv) + 3 * TRW) * 4;  // s_val | long sums (doubles) | window
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     } else if (kern == 1 || kern == 3) {
-        const size_t lds = std::max((size_t)(wide_rows(K, kBatchRI) + kWideHalo) * K * 4,
-                                    (size_t)wide_rows(K, kBatchRI) * K * 8);
+        const int trw = wide_rows(K, kBatchRI, (int)sizeof(XT));
+        const size_t lds = std::max((size_t)(trw + kWideHalo) * K * 4, (size_t)trw * K * 8);
         if (kern == 3) {
             if (K == 2) batch_wide_kernel<2, XT, true><<<grid, kWG, lds, s>>>(D, UB, nd, units);
             else if (K == 4) batch_wide_kernel<4, XT, true><<<grid, kWG, lds, s>>>(D, UB, nd, units);
@@ -189,7 +189,7 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
 // chunk c0..c0+K of request r (kv of its columns exist): its batch class
 bool chunk_aligned(const tc_request &r, int K, int c0)
 {
-    const int es = elem_size(r.dtype), cpl = std::min(K, 4);
+    const int es = elem_size(r.dtype), cpl = wide_cpl(K, es);
     const uintptr_t xa = (uintptr_t)r.X + (uintptr_t)c0 * es, ya = (uintptr_t)(r.Y + c0);
     return r.ldx % cpl == 0 && r.ldy % cpl == 0 && xa % ((uintptr_t)cpl * es) == 0 &&
            ya % ((uintptr_t)cpl * 4) == 0;
@@ -274,7 +274,7 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
             w.ldy = r.ldy;
             w.fast = fast ? 1 : 0;
             w.kv = kv;
-            const int trw = batch_tile_rows(cl / 12, K);
+            const int trw = batch_tile_rows(cl / 12, K, elem_size(r.dtype));
             w.unit_begin = (int)units[cl];
             UB[fill[cl]] = w.unit_begin;
             units[cl] += (v.M + trw - 1) / trw;

@@ -64,8 +64,8 @@ constexpr int kStagedK = 4;
 
 inline bool wide_k(int K) { return K == 2 || K == 4 || K == 8 || K == 16 || K == 32 || K == 64; }
 
-// rows of a wide-rows tile with RI rows per lane group
-inline int wide_rows(int K, int RI) { return (kWG / (K / std::min(K, 4))) * RI; }
+// rows of a wide-rows tile with RI rows per lane group (x elements of esize bytes)
+inline int wide_rows(int K, int RI, int esize) { return (kWG / (K / wide_cpl(K, esize))) * RI; }
 
 // A column count with no wide kernel of its own (3, 5, 6, 7) runs as one
 // masked pass of this wide width (0: none) when X and Y rows split into

@@ -707,6 +707,34 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
     }
 }
 
+// Q consecutive floats of LDS (Q = 2, 4 or 8: one or two vector accesses)
+template <int Q>
+__device__ __forceinline__ void lds_get(const float *s, float (&o)[Q])
+{
+    if constexpr (Q == 2) {
+        const float2 v = *(const float2 *)s;
+        o[0] = v.x, o[1] = v.y;
+    } else {
+#pragma unroll
+        for (int h = 0; h < Q; h += 4) {
+            const float4 v = *(const float4 *)(s + h);
+            o[h] = v.x, o[h + 1] = v.y, o[h + 2] = v.z, o[h + 3] = v.w;
+        }
+    }
+}
+template <int Q>
+__device__ __forceinline__ void lds_put(float *d, const float (&v)[Q])
+{
+    if constexpr (Q == 2) {
+        *(float2 *)d = make_float2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int h = 0; h < Q; h += 4) *(float4 *)(d + h) = make_float4(v[h], v[h + 1], v[h + 2], v[h + 3]);
+    }
+}
+
+// x row segment of Q elements as floats: one vector load (Q * sizeof(XT)
+// bytes: 8 or 16 for 4-byte x, 4, 8 or 16 for 2-byte x)
 template <typename XT, int Q>
 __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
 {
@@ -720,7 +748,10 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
         }
     } else {
         uint32_t w[Q / 2];
-        if constexpr (Q == 4) {
+        if constexpr (Q == 8) {
+            const uint4 v = *(const uint4 *)p;
+            w[0] = v.x, w[1] = v.y, w[2] = v.z, w[3] = v.w;
+        } else if constexpr (Q == 4) {
             const uint2 v = *(const uint2 *)p;
             w[0] = v.x, w[1] = v.y;
         } else {
@@ -753,9 +784,14 @@ __device__ __forceinline__ void load_qm(const XT *__restrict__ p, int nv, bool l
     }
 }
 
+// Columns a lane owns in the wide-rows tiles: 4 (a 16-byte FP32 vector, an
+// 8-byte BF16/FP16 one), or 8 for 2-byte x when K >= 8 (a 16-byte vector: half
+// the lanes per row, half the vector memory instructions)
+__host__ __device__ constexpr int wide_cpl(int K, int esize) { return esize == 2 && K >= 8 ? 8 : (K < 4 ? K : 4); }
+
 // ---- k > 1: wide rows ----
 // A row belongs to a group of LPR = K/CPL lanes; lane q of the group owns the
-// CPL = min(K, 4) consecutive columns q*CPL.., so each x row segment the row
+// CPL = wide_cpl(K, sizeof(XT)) consecutive columns q*CPL.., so each x row segment the row
 // touches (a diagonal term or a remainder entry) is one CPL-wide vector load
 // per lane and y leaves as one vector store per lane. Consecutive groups own
 // consecutive rows (y is written contiguously when ldy = K). Rows are
@@ -775,7 +811,7 @@ template <int K, typename XT, int RI, bool LONG, bool MASK = false>
 __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
                                           int ldy, int t, float *smem, int kv = K)
 {
-    constexpr int CPL = K < 4 ? K : 4;
+    constexpr int CPL = wide_cpl(K, (int)sizeof(XT));
     constexpr int LPR = K / CPL;
     constexpr int GPW = kWG / LPR;  // row groups per workgroup
     constexpr int TRW = GPW * RI;   // rows per tile
@@ -848,10 +884,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                         for (int u = 0; u < 4; u++) {
                             const int v = b + u * kWG + tid;
                             if (v < nw) {
-                                float *d = smem + (v / LPR) * K + (v % LPR) * CPL;
-                                if constexpr (CPL == 4)
-                                    *(float4 *)d = make_float4(tv[u][0], tv[u][1], tv[u][2], tv[u][3]);
-                                else *(float2 *)d = make_float2(tv[u][0], tv[u][1]);
+                                lds_put<CPL>(smem + (v / LPR) * K + (v % LPR) * CPL, tv[u]);
                             }
                         }
                     }
@@ -886,16 +919,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
 #pragma unroll
                                 for (int u = 0; u < DU; u++)
 #pragma unroll
-                                    for (int i = 0; i < RI; i++) {
-                                        const float *sp = smem + base[i] + o[u];
-                                        if constexpr (CPL == 4) {
-                                            const float4 t = *(const float4 *)sp;
-                                            q[u][i][0] = t.x, q[u][i][1] = t.y, q[u][i][2] = t.z, q[u][i][3] = t.w;
-                                        } else {
-                                            const float2 t = *(const float2 *)sp;
-                                            q[u][i][0] = t.x, q[u][i][1] = t.y;
-                                        }
-                                    }
+                                    for (int i = 0; i < RI; i++) lds_get<CPL>(smem + base[i] + o[u], q[u][i]);
 #pragma unroll
                                 for (int u = 0; u < DU; u++) {
                                     if (u >= nu) break;
@@ -918,18 +942,13 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                             const int r = u0 + GPW * i + g;
                             const int c = r + shift + o;
                             if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
-                                const float *sp = smem + (c - w0) * K + c0;
                                 const int j = n - sh.ob;
                                 const uint32_t w = j < 32 ? plo[i] : phi[i];
-                                auto kp = [&](float v) { return PM ? keep_if(v, w, j & 31) : v; };
-                                if constexpr (CPL == 4) {
-                                    const float4 q = *(const float4 *)sp;
-                                    accum(acc[i * 4], kp(q.x)), accum(acc[i * 4 + 1], kp(q.y));
-                                    accum(acc[i * 4 + 2], kp(q.z)), accum(acc[i * 4 + 3], kp(q.w));
-                                } else {
-                                    const float2 q = *(const float2 *)sp;
-                                    accum(acc[i * 2], kp(q.x)), accum(acc[i * 2 + 1], kp(q.y));
-                                }
+                                float q[CPL];
+                                lds_get<CPL>(smem + (c - w0) * K + c0, q);
+#pragma unroll
+                                for (int m = 0; m < CPL; m++)
+                                    accum(acc[i * CPL + m], PM ? keep_if(q[m], w, j & 31) : q[m]);
                             }
                         }
                     }
@@ -1012,9 +1031,14 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     if (m < nv) yp[m] = (float)acc[i * CPL + m];
                 continue;
             }
-            if constexpr (CPL == 4)
-                st_nt4(yp, (float)acc[i * 4], (float)acc[i * 4 + 1], (float)acc[i * 4 + 2], (float)acc[i * 4 + 3]);
-            else st_nt2(yp, (float)acc[i * 2], (float)acc[i * 2 + 1]);
+            if constexpr (CPL == 2) {
+                st_nt2(yp, (float)acc[i * 2], (float)acc[i * 2 + 1]);
+            } else {
+#pragma unroll
+                for (int h = 0; h < CPL; h += 4)
+                    st_nt4(yp + h, (float)acc[i * CPL + h], (float)acc[i * CPL + h + 1], (float)acc[i * CPL + h + 2],
+                           (float)acc[i * CPL + h + 3]);
+            }
         }
     }
 }

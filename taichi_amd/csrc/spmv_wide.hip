@@ -18,7 +18,7 @@ __global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__r
 template <int K, typename XT, int RI>
 int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int grid, int kv, hipStream_t s)
 {
-    constexpr int TRW = (kWG / (K / (K < 4 ? K : 4))) * RI;
+    constexpr int TRW = (kWG / (K / wide_cpl(K, (int)sizeof(XT)))) * RI;
     const int ntiles = (A.M + TRW - 1) / TRW;
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
@@ -45,7 +45,8 @@ template <int K, typename XT>
 int launch_wide_k(const MatView &A, const void *X, int ldx, float *Y, int ldy, const tc_config &c, int kv,
                   hipStream_t s)
 {
-    if (c.rows_per_tile == wide_rows(K, 1)) return launch_wide_t<K, XT, 1>(A, X, ldx, Y, ldy, c.grid, kv, s);
+    if (c.rows_per_tile == wide_rows(K, 1, (int)sizeof(XT)))
+        return launch_wide_t<K, XT, 1>(A, X, ldx, Y, ldy, c.grid, kv, s);
     return launch_wide_t<K, XT, 2>(A, X, ldx, Y, ldy, c.grid, kv, s);
 }
 
@@ -67,8 +68,8 @@ int launch_wide_x(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
 bool wide_applies(int K, const void *X, int ldx, const float *Y, int ldy, int esize, const tc_config &c)
 {
     if (!wide_k(K) || (c.flags & kStagedK)) return false;
-    if (c.rows_per_tile != wide_rows(K, 1) && c.rows_per_tile != wide_rows(K, 2)) return false;
-    const int cpl = std::min(K, 4);
+    if (c.rows_per_tile != wide_rows(K, 1, esize) && c.rows_per_tile != wide_rows(K, 2, esize)) return false;
+    const int cpl = wide_cpl(K, esize);
     return ldx % cpl == 0 && ldy % cpl == 0 && (uintptr_t)X % ((uintptr_t)cpl * esize) == 0 &&
            (uintptr_t)Y % ((uintptr_t)cpl * 4) == 0;
 }
