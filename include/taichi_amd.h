@@ -209,9 +209,12 @@ int tc_spmv(tc_matrix *A, const void *X, int ldx, int dtype, float *Y, int ldy,
 
 /* launch configuration (no reference counterpart; the reference's TPV tier,
    TaiChi.cu:1390-1394, plays this role): rows per workgroup tile (k = 1:
-   256, 512, 1024, 2048 or 4096 = 1..16 rows per lane; k > 1: 256*P/k for P
-   in 1, 2, 4), LDS value capacity, persistent grid size (0 = one workgroup
-   per tile), kernel choice. Results do not depend on the configuration. */
+   256, 512, 1024, 2048 or 4096 = 1..16 rows per lane; k > 1, wide rows:
+   256*c/k*R for R in 1, 2 rows per lane group, c the columns a lane owns --
+   4, or 8 for BF16/FP16 x at k >= 8 (2 for k = 2); k > 1, staged tiles
+   (flag bit 2): 256*P/k for P in 1, 2, 4), LDS value capacity, persistent
+   grid size (0 = one workgroup per tile), kernel choice. Configurations are
+   kept per (k, x type). Results do not depend on the configuration. */
 typedef struct tc_config {
     int rows_per_tile;   /* power of two, 1..2048; 0 = default for k */
     int lds_values;      /* LDS staging capacity in floats (512..8192); 0 = default */
