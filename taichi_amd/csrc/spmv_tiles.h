@@ -784,6 +784,31 @@ __device__ __forceinline__ void load_qm(const XT *__restrict__ p, int nv, bool l
     }
 }
 
+// The first nv (0 < nv < Q) of Q values to yp (Q-aligned) in aligned pieces
+// of 4, 2 and 1: at most three stores, nothing past column nv. Indices are
+// constants in every branch, so f stays in registers.
+template <int Q>
+__device__ __forceinline__ void store_first(float *yp, const float (&f)[Q], int nv)
+{
+    if constexpr (Q >= 8) {
+        if (nv >= 4) {
+            st_nt4(yp, f[0], f[1], f[2], f[3]);
+            if (nv == 5) st_nt(yp + 4, f[4]);
+            else if (nv >= 6) st_nt2(yp + 4, f[4], f[5]);
+            if (nv == 7) st_nt(yp + 6, f[6]);
+            return;
+        }
+    }
+    if constexpr (Q >= 4) {
+        if (nv >= 2) {
+            st_nt2(yp, f[0], f[1]);
+            if (nv == 3) st_nt(yp + 2, f[2]);
+            return;
+        }
+    }
+    st_nt(yp, f[0]);
+}
+
 // Columns a lane owns in the wide-rows tiles: 4 (a 16-byte FP32 vector, an
 // 8-byte BF16/FP16 one), or 8 for 2-byte x when K >= 8 (a 16-byte vector: half
 // the lanes per row, half the vector memory instructions)
@@ -1026,9 +1051,10 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             if (r >= A.M) continue;
             float *yp = Y + (size_t)r * ldy + c0;
             if (MASK && nv < CPL) {
+                float f[CPL];
 #pragma unroll
-                for (int m = 0; m < CPL; m++)
-                    if (m < nv) yp[m] = (float)acc[i * CPL + m];
+                for (int m = 0; m < CPL; m++) f[m] = (float)acc[i * CPL + m];
+                store_first<CPL>(yp, f, nv);
                 continue;
             }
             if constexpr (CPL == 2) {
