@@ -122,3 +122,86 @@ def test_batched_equals_alone(dev):
     torch.cuda.synchronize()
     for (_, _, Yb), Ya in zip(items, alone):
         assert torch.equal(Yb, Ya)
+
+
+def _corners_format(seed=3):
+    """Hand-built TaiChi_new for the layout planner's corners:
+    shape 0, rows [100, 2100), 40 offsets stored out of order, many zero cells
+    cancelled (gets a bitmap of two words a row); shape 1, rows [2500, 4500),
+    70 offsets (too many for a bitmap: its cancelled cells stay entries);
+    shape 2, rows [1000, 1500) inside shape 0's rows, 4 offsets, few zero cells
+    (no bitmap). Rows also hold positive entries, a sign-0 entry on no
+    diagonal, and some cancelled cells listed twice (the second stays)."""
+    M = N = 6000
+    rng = np.random.default_rng(seed)
+    offs0 = rng.permutation(np.arange(-20, 20)).astype(np.int32)
+    offs1 = np.arange(-35, 35, dtype=np.int32)
+    offs2 = np.array([0, 1, 2, 3], np.int32)
+    xs, xe, ys = [100, 2500, 1000], [2100, 4500, 1500], [100, 2600, 3000]
+    f = T.Format(M, N)
+    f.xStart, f.xStop, f.yStart = (np.array(v, np.int32) for v in (xs, xe, ys))
+    f.nDiasPtr = np.array([0, 40, 110, 114], np.int32)
+    f.neg_offsets = np.concatenate([offs0, offs1, offs2])
+    rows, absorbed, pos, neg = [], 0, 0, 0
+    for r in range(M):
+        ent = []
+        for c in rng.integers(0, N, int(rng.integers(0, 4))):
+            ent.append((int(c) << 1) | 1)
+        for s, (offs, p) in enumerate(((offs0, 0.3), (offs1, 0.1), (offs2, 0.2))):
+            if not (xs[s] <= r < xe[s]):
+                continue
+            for o in offs:
+                c = r - xs[s] + ys[s] + int(o)
+                if ys[s] <= c < ys[s] + (xe[s] - xs[s]) and rng.random() < p:
+                    ent.append(c << 1)
+                    if s == 0:
+                        absorbed += 1
+                        if rng.random() < 0.05:
+                            ent.append(c << 1)  # listed twice: only the first is absorbed
+        if r % 97 == 0:
+            ent.append(((r + 3000) % N) << 1)  # on no diagonal
+        ent = [ent[i] for i in rng.permutation(len(ent))]
+        pos += sum(e & 1 for e in ent)
+        neg += sum(1 - (e & 1) for e in ent)
+        rows.append(np.array(ent, np.int32))
+    rp = np.zeros(M + 1, np.int32)
+    rp[1:] = np.cumsum([len(e) for e in rows])
+    f.csrRow, f.csrCol = rp, np.concatenate(rows).astype(np.int32)
+    f.dia_shapes, f.num_negs = 3, 114
+    f.dense_nnz = 1  # any positive count marks the columns as encoded
+    f.sparse_nnz, f.zero_elements = pos, neg
+    f.row_start, f.row_stop = 0, M
+    return f, absorbed
+
+
+def test_hand_made_layout_corners(dev):
+    f, absorbed = _corners_format()
+    A = T.Matrix(f)
+    st = A.stats()
+    assert st["bitmap_shapes"] == 1
+    assert st["holes"] == absorbed
+    assert st["holes"] + st["remainder"] == int(f.csrRow[-1])
+    assert T.footprint(f) == st["device_bytes"]
+    rng = np.random.default_rng(5)
+    xi = rng.integers(0, 8, f.N).astype(np.float32)
+    want = O.taichi_spmv(f, xi, np.float32)
+    for k in (1, 4, 8):
+        X = torch.from_numpy(np.repeat(xi[:, None], k, 1)).to(dev)
+        Y = torch.full((f.M, k), float("nan"), device=dev)
+        A.spmv(X if k > 1 else X[:, 0].contiguous(), Y if k > 1 else Y[:, 0])
+        torch.cuda.synchronize()
+        for c in range(k):
+            assert np.array_equal(Y[:, c].cpu().numpy(), want), (k, c)
+    xf = rng.standard_normal(f.N)
+    y = torch.empty(f.M, device=dev)
+    A.spmv(torch.from_numpy(xf.astype(np.float32)).to(dev), y)
+    torch.cuda.synchronize()
+    x64 = xf.astype(np.float32).astype(np.float64)
+    y64 = O.taichi_spmv(f, x64)
+    fa = T.Format(f.M, f.N)
+    for name in T.Format.FIELDS + T.Format.ARRAYS:
+        setattr(fa, name, getattr(f, name))
+    fa.csrCol = f.csrCol | 1  # every term with a plus sign: the bound's denominator
+    den = np.maximum(np.abs(y64), O.taichi_spmv(fa, np.abs(x64)))
+    den[den == 0] = 1
+    assert float(np.max(np.abs(y.cpu().numpy() - y64) / den)) <= 1e-6
