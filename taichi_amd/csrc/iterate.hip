@@ -6,14 +6,17 @@
 // for a page without out-links (dangling). From r_0 = 1/n, each step is
 //   z   = r (.) inv_deg                    (scaled ranks, the SpMV's x)
 //   D   = sum of r over dangling pages     (their mass is spread uniformly)
-//   y   = A z                              (tc_spmv)
+//   y   = A z
 //   r'  = (1 - alpha)/n + alpha (y + D/n)
-// One fused kernel per step reads y, writes the next z (and r' on the last
-// step) and one partial sum of the dangling mass per block; the next step's
-// blocks each fold the previous partials in the same fixed order (two
-// partial buffers alternate), so no separate reduction launch is needed. All
-// sums have fixed orders, so the ranks are identical eagerly and replayed
-// from the graph.
+// A step is the k = 1 rows tiles of the SpMV (spmv_tiles.h) whose end turns
+// each row's sum y_i straight into r'_i and z'_i = r'_i inv_deg_i (y is
+// never stored; z alternates between two buffers) and sums r' over the
+// dangling rows, one partial per block; then a one-block launch folds the
+// partials in index order into D' for the next step. (The last block of the
+// step doing the fold instead, found by an atomic count, measured 8x slower:
+// thousands of device-scope atomics on one word serialise.) Every sum has a
+// fixed order, so the ranks are identical eagerly and replayed from the
+// graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,178 +27,175 @@
 namespace {
 
 constexpr int kWG = 256;
-constexpr int kParts = 1024;  // blocks of the fused update (grid-stride), one partial each
+constexpr int kInitBlocks = 1024;  // blocks of the first step (grid-stride)
 
-// fixed-order sum of a workgroup's 256 values (tree over LDS)
-__device__ float block_sum(float v, float *sh)
+// this wavefront's partial d to part[4 * blockIdx.x + wave]: an xor butterfly
+// (both lanes of a pair add the same two values), no block barrier
+__device__ void wave_partial(float d, float *part)
 {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int s = kWG / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] += sh[t + s];
-        __syncthreads();
-    }
-    const float r = sh[0];
-    __syncthreads();
-    return r;
-}
-
-// Step c of the iteration (c = 0: r = 1/n, no SpMV before it). Reads the
-// partials of step c-1 from prev, writes its own to part; writes r only when
-// write_r is set (the last step), z always. VEC: all four vectors are 16-byte
-// aligned and each block takes chunks of kChunk elements, every lane issuing
-// kU float4 loads of y and of inv_deg before it uses any (enough bytes in
-// flight to stream HBM); the n % 4 tail is done by block 0. Scalar otherwise.
-constexpr int kU = 4;
-constexpr int kChunk = kWG * 4 * kU;
-
-__device__ inline float rank_of(int first, float inv_n, float base, float alpha, float y, float spread)
-{
-    return first ? inv_n : base + alpha * (y + spread);
-}
-
-// Sum of the previous step's partials (at most kParts), the same value in
-// every lane of every wave: lane l loads partials l, l + 64, ... all at once
-// and adds them in that order, then an xor butterfly (both lanes of a pair add
-// the same two values, so they stay identical).
-__device__ inline float fold_partials(const float *__restrict__ prev, int nprev)
-{
-    const int lane = threadIdx.x & 63;
-    float v[kParts / 64];
-#pragma unroll
-    for (int j = 0; j < kParts / 64; j++) {
-        const int i = j * 64 + lane;
-        v[j] = i < nprev ? prev[i] : 0.0f;
-    }
-    float d = 0.0f;
-#pragma unroll
-    for (int j = 0; j < kParts / 64; j++) d += v[j];
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) d += __shfl_xor(d, m, 64);
-    return d;
+    if ((threadIdx.x & 63) == 0) part[blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)] = d;
 }
 
-__device__ inline void load_chunk(long long c, long long n4, int first, const float4 *__restrict__ y4,
-                                  const float4 *__restrict__ w4, float4 *yv, float4 *wv)
+// D = the sum of part[0..n) in a fixed order (one block of kFold lanes):
+// lane t adds t, t + kFold, ... (loads issued kFoldU at a time), then the
+// wavefront butterflies and the wavefront sums in order
+constexpr int kFold = 1024, kFoldU = 8;
+__global__ __launch_bounds__(kFold) void pr_fold(const float *__restrict__ part, int n, float *__restrict__ D)
 {
+    __shared__ float sw[kFold / 64];
+    float a = 0.0f;
+    for (int b = 0; b < n; b += kFold * kFoldU) {
+        float v[kFoldU];
 #pragma unroll
-    for (int u = 0; u < kU; u++) {
-        const long long i = c + u * kWG + threadIdx.x;
-        if (i < n4) {
-            yv[u] = first ? make_float4(0.f, 0.f, 0.f, 0.f) : y4[i];
-            wv[u] = w4[i];
+        for (int u = 0; u < kFoldU; u++) {
+            const int i = b + u * kFold + threadIdx.x;
+            v[u] = i < n ? part[i] : 0.0f;
         }
+#pragma unroll
+        for (int u = 0; u < kFoldU; u++) a += v[u];
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) a += __shfl_xor(a, m, 64);
+    if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kFold / 64; w++) t += sw[w];
+        *D = t;
     }
 }
 
-template <bool VEC>
-__global__ __launch_bounds__(kWG) void pr_update(int n, float alpha, int first, int write_r,
-                                                 const float *__restrict__ y, const float *__restrict__ inv_deg,
-                                                 const float *__restrict__ prev, int nprev, float *__restrict__ r,
-                                                 float *__restrict__ z, float *__restrict__ part)
+// step 0: r = 1/n, z = r inv_deg, D = the dangling mass of r (r written only
+// when no iteration follows)
+__global__ __launch_bounds__(kWG) void pr_init(int n, int write_r, const float *__restrict__ inv_deg,
+                                               float *__restrict__ r, float *__restrict__ z, float *part)
 {
-    __shared__ float sh[kWG];
     const float inv_n = 1.0f / (float)n;
-    const float base = (1.0f - alpha) * inv_n;
     float d = 0.0f;
-    if constexpr (VEC) {
-        const long long n4 = n >> 2;
-        const float4 *y4 = (const float4 *)y;
-        const float4 *w4 = (const float4 *)inv_deg;
-        const long long step = (long long)gridDim.x * (kChunk / 4);
-        long long c = (long long)blockIdx.x * (kChunk / 4);
-        float4 yv[kU], wv[kU];
-        load_chunk(c, n4, first, y4, w4, yv, wv);  // in flight while the partials are folded
-        const float spread = first ? 0.0f : fold_partials(prev, nprev) * inv_n;
-        for (; c < n4; c += step) {
-#pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const long long i = c + u * kWG + threadIdx.x;
-                if (i < n4) {
-                    float4 rv, zv;
-                    rv.x = rank_of(first, inv_n, base, alpha, yv[u].x, spread);
-                    rv.y = rank_of(first, inv_n, base, alpha, yv[u].y, spread);
-                    rv.z = rank_of(first, inv_n, base, alpha, yv[u].z, spread);
-                    rv.w = rank_of(first, inv_n, base, alpha, yv[u].w, spread);
-                    zv.x = rv.x * wv[u].x;
-                    zv.y = rv.y * wv[u].y;
-                    zv.z = rv.z * wv[u].z;
-                    zv.w = rv.w * wv[u].w;
-                    if (write_r) ((float4 *)r)[i] = rv;
-                    ((float4 *)z)[i] = zv;
-                    if (wv[u].x == 0.0f) d += rv.x;
-                    if (wv[u].y == 0.0f) d += rv.y;
-                    if (wv[u].z == 0.0f) d += rv.z;
-                    if (wv[u].w == 0.0f) d += rv.w;
-                }
-            }
-            if (c + step < n4) load_chunk(c + step, n4, first, y4, w4, yv, wv);
-        }
-        if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
-            const int i = (int)(n4 << 2) + threadIdx.x;
-            const float ri = rank_of(first, inv_n, base, alpha, first ? 0.0f : y[i], spread);
-            const float w = inv_deg[i];
-            if (write_r) r[i] = ri;
-            z[i] = ri * w;
-            if (w == 0.0f) d += ri;
-        }
-    } else {
-        const float spread = first ? 0.0f : fold_partials(prev, nprev) * inv_n;
-        for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
-            const float ri = rank_of(first, inv_n, base, alpha, first ? 0.0f : y[i], spread);
-            const float w = inv_deg[i];
-            if (write_r) r[i] = ri;
-            z[i] = ri * w;
-            if (w == 0.0f) d += ri;
-        }
+    for (int i = blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
+        const float w = inv_deg[i];
+        if (write_r) r[i] = inv_n;
+        z[i] = inv_n * w;
+        if (w == 0.0f) d += inv_n;
     }
-    d = block_sum(d, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = d;
+    wave_partial(d, part);
 }
 
-// work layout: y[n] | z[n] | part[2][kParts], each 256-byte aligned
+// what a row's sum becomes in a step: r'_i, z'_i and the dangling sum
+struct PrEnd {
+    static constexpr bool kStore = false;
+    const float *inv_deg;
+    float *z, *r;
+    float base, alpha, spread;
+    int write_r;
+    float &d;
+    __device__ void operator()(int i, double sum) const
+    {
+        const float ri = base + alpha * ((float)sum + spread);
+        const float w = inv_deg[i];
+        if (write_r) r[i] = ri;
+        z[i] = ri * w;
+        if (w == 0.0f) d += ri;
+    }
+};
+
+// one step: z' and r' from A zin with the previous step's D; one partial of
+// the dangling mass per block
+template <int RPT, int OW, bool FAST, bool LONG>
+__global__ __launch_bounds__(kWG) void pr_step(tcd::MatView A, const float *__restrict__ zin, float *__restrict__ zout,
+                                               float *__restrict__ r, const float *__restrict__ inv_deg, float alpha,
+                                               int write_r, const float *Dprev, float *part, int ntiles, int capv)
+{
+    extern __shared__ float smem[];
+    const float inv_n = 1.0f / (float)A.M;
+    float d = 0.0f;
+    const PrEnd end{inv_deg, zout, r, (1.0f - alpha) * inv_n, alpha, *Dprev * inv_n, write_r, d};
+    tcd::rows_tile<float, RPT, OW, FAST, LONG, true, PrEnd>(A, zin, 1, nullptr, 1,
+                                                            tcd::xcd_slot(blockIdx.x, ntiles), smem, capv, end);
+    wave_partial(d, part);
+}
+
+template <int RPT, int OW, bool FAST, bool LONG>
+void launch_step_k(const tcd::MatView &A, const float *zin, float *zout, float *r, const float *inv_deg,
+                   float alpha, int write_r, const float *Dp, float *part, int ntiles, int capv, size_t lds,
+                   hipStream_t s)
+{
+    pr_step<RPT, OW, FAST, LONG><<<ntiles, kWG, lds, s>>>(A, zin, zout, r, inv_deg, alpha, write_r, Dp, part, ntiles,
+                                                          capv);
+}
+
+// one step and the fold of its partials into *Dn
+template <int RPT>
+int launch_step_t(const tcd::MatView &A, const float *zin, float *zout, float *r, const float *inv_deg,
+                  float alpha, int write_r, const float *Dp, float *Dn, float *part, int capv, hipStream_t s)
+{
+    constexpr int TRW = tcd::kWG * RPT;
+    const int ntiles = (A.M + TRW - 1) / TRW;
+    const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1 && A.sh0.pm < 0;
+    const bool lng = A.L > 0;
+    const size_t lds = ((size_t)tcd::lsum_at(capv) + (lng ? 2 * TRW : 0) + (fast ? TRW : 0)) * 4;
+    if (lds + kWG * 4 > tcd::max_lds_bytes())
+        return tc::fail(TC_ERR_UNSUPPORTED, "pagerank: tiles of %d rows with %d LDS values need %zu B", TRW, capv, lds);
+#define TC_STEP(OW, F, L) \
+    launch_step_k<RPT, OW, F, L>(A, zin, zout, r, inv_deg, alpha, write_r, Dp, part, ntiles, capv, lds, s)
+    if (A.ow == 1) {
+        if (fast) lng ? TC_STEP(1, true, true) : TC_STEP(1, true, false);
+        else lng ? TC_STEP(1, false, true) : TC_STEP(1, false, false);
+    } else {
+        if (fast) lng ? TC_STEP(2, true, true) : TC_STEP(2, true, false);
+        else lng ? TC_STEP(2, false, true) : TC_STEP(2, false, false);
+    }
+#undef TC_STEP
+    pr_fold<<<1, kFold, 0, s>>>(part, ntiles * (kWG / 64), Dn);
+    return TC_OK;
+}
+
+// work layout: z[2][n] | part[nparts] | D[2], each 256-byte aligned
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// one partial per wavefront of the first step's or a step's grid
+size_t nparts_for(int n) { return (kWG / 64) * std::max((size_t)kInitBlocks, ((size_t)n + 255) / 256); }
+
 struct Work {
-    float *y, *z, *part[2];
+    float *z[2], *part, *D;
 };
 
 Work carve(void *work, int n)
 {
     char *p = (char *)work;
     Work w;
-    w.y = (float *)p;
+    w.z[0] = (float *)p;
     p += align256((size_t)n * 4);
-    w.z = (float *)p;
+    w.z[1] = (float *)p;
     p += align256((size_t)n * 4);
-    w.part[0] = (float *)p;
-    p += align256((size_t)kParts * 4);
-    w.part[1] = (float *)p;
+    w.part = (float *)p;
+    p += align256(nparts_for(n) * 4);
+    w.D = (float *)p;
     return w;
 }
 
-int grid_for(int n, bool vec)
-{
-    const int per = vec ? kChunk : kWG;
-    return std::max(1, std::min(kParts, (n + per - 1) / per));
-}
-
-bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
-
-// the first step's update (r = 1/n) then `iters` iterations, all on s
+// the first step (r = 1/n) then `iters` iterations, all on s. The rows tile
+// size follows the matrix's k = 1 configuration (256 rows, else 512) and its
+// LDS stage.
 int enqueue_pagerank(tc_matrix *A, int n, const float *inv_deg, float alpha, int iters, float *r, const Work &w,
                      hipStream_t s)
 {
-    const bool vec = aligned16(inv_deg) && aligned16(r) && aligned16(w.y) && aligned16(w.z);
-    const int G = grid_for(n, vec);
-    auto update = vec ? pr_update<true> : pr_update<false>;
-    update<<<G, kWG, 0, s>>>(n, alpha, 1, iters == 0, w.y, inv_deg, w.part[1], G, r, w.z, w.part[0]);
+    const tc_config c = tcd::get_cfg(A, 1, TC_F32);
+    const int capv = std::max(c.lds_values, 64);
+    const bool one = c.rows_per_tile == 256;
+    const int G = std::max(1, std::min(kInitBlocks, (n + kWG - 1) / kWG));
+    pr_init<<<G, kWG, 0, s>>>(n, iters == 0, inv_deg, r, w.z[0], w.part);
+    pr_fold<<<1, kFold, 0, s>>>(w.part, G * (kWG / 64), &w.D[0]);
     for (int it = 1; it <= iters; it++) {
-        int rc = tc_spmv(A, w.z, 1, TC_F32, w.y, 1, 1, s);
+        const float *zin = w.z[(it - 1) & 1];
+        float *zout = w.z[it & 1];
+        const float *Dp = &w.D[(it - 1) & 1];
+        float *Dn = &w.D[it & 1];
+        const int rc = one ? launch_step_t<1>(A->v, zin, zout, r, inv_deg, alpha, it == iters, Dp, Dn, w.part, capv, s)
+                           : launch_step_t<2>(A->v, zin, zout, r, inv_deg, alpha, it == iters, Dp, Dn, w.part, capv, s);
         if (rc != TC_OK) return rc;
-        update<<<G, kWG, 0, s>>>(n, alpha, 0, it == iters, w.y, inv_deg, w.part[(it - 1) & 1], G, r, w.z,
-                                 w.part[it & 1]);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TC_OK : tc::fail(TC_ERR_HIP, "pagerank launch: %s", hipGetErrorString(e));
@@ -206,7 +206,7 @@ int enqueue_pagerank(tc_matrix *A, int n, const float *inv_deg, float alpha, int
 extern "C" size_t tc_pagerank_work_bytes(int n)
 {
     if (n < 0) return 0;
-    return 2 * align256((size_t)n * 4) + 2 * align256((size_t)kParts * 4);
+    return 2 * align256((size_t)n * 4) + align256(nparts_for(n) * 4) + align256(2 * 4);
 }
 
 // checks shared by the eager and the captured form

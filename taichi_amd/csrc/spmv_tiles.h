@@ -556,9 +556,18 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
 // PIPE (general path only): prefetch the next batch's column indices and sum
 // rows with four LDS reads in flight; the batched kernel, whose registers
 // serve every matrix kind in one launch, leaves it off.
-template <typename XT, int RPT, int OW, bool FAST, bool LONG, bool PIPE = true>
+// The default end of a rows tile: y[row] = (float)sum.
+struct StoreY {
+    static constexpr bool kStore = true;
+    __device__ void operator()(int, double) const {}
+};
+
+// EPI: what a finished row's FP64 sum becomes; StoreY writes it to Y, any
+// other functor is called as epi(row, sum) instead (the iterative driver
+// turns it into the next iterate there).
+template <typename XT, int RPT, int OW, bool FAST, bool LONG, bool PIPE = true, class EPI = StoreY>
 __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
-                                          int ldy, int t, float *smem, int capv)
+                                          int ldy, int t, float *smem, int capv, const EPI &epi = EPI{})
 {
     constexpr int TRW = kWG * RPT;
     float *s_val = smem;
@@ -702,7 +711,8 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             if (!pv[j]) continue;
-            st_nt(&Y[(size_t)(u0 + prow[j]) * ldy], (float)acc[j]);
+            if constexpr (EPI::kStore) st_nt(&Y[(size_t)(u0 + prow[j]) * ldy], (float)acc[j]);
+            else epi(u0 + prow[j], acc[j]);
         }
     }
 }
