@@ -41,7 +41,7 @@ __device__ void wave_partial(float d, float *part)
 // D = the sum of part[0..n) in a fixed order (one block of kFold lanes):
 // lane t adds t, t + kFold, ... (loads issued kFoldU at a time), then the
 // wavefront butterflies and the wavefront sums in order
-constexpr int kFold = 1024, kFoldU = 8;
+constexpr int kFold = 1024, kFoldU = 32;
 __global__ __launch_bounds__(kFold) void pr_fold(const float *__restrict__ part, int n, float *__restrict__ D)
 {
     __shared__ float sw[kFold / 64];
