@@ -945,26 +945,41 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                             uint32_t w[RI];
 #pragma unroll
                             for (int i = 0; i < RI; i++) w[i] = seg ? phi[i] : plo[i];
-                            for (int n = nb; n < ne; n += DU) {
-                                const int nu = min(DU, ne - n);  // uniform
+                            // whole groups of DU offsets with no exits: one
+                            // scalar load brings the group's offsets (an exit
+                            // per offset makes the compiler wait on a scalar
+                            // load for each); each offset's LDS reads and adds
+                            // then stay together (sched_barrier), so only
+                            // RI * CPL x values are live and occupancy holds.
+                            // The rest run one offset at a time.
+                            int n = nb;
+                            for (; n + DU <= ne; n += DU) {
                                 int o[DU];
 #pragma unroll
-                                for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + min(u, nu - 1)) * K;
-                                float q[DU][RI][CPL];
-#pragma unroll
-                                for (int u = 0; u < DU; u++)
-#pragma unroll
-                                    for (int i = 0; i < RI; i++) lds_get<CPL>(smem + base[i] + o[u], q[u][i]);
+                                for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * K;
 #pragma unroll
                                 for (int u = 0; u < DU; u++) {
-                                    if (u >= nu) break;
+                                    float q[RI][CPL];
+#pragma unroll
+                                    for (int i = 0; i < RI; i++) lds_get<CPL>(smem + base[i] + o[u], q[i]);
 #pragma unroll
                                     for (int i = 0; i < RI; i++)
 #pragma unroll
                                         for (int m = 0; m < CPL; m++)
-                                            accum(acc[i * CPL + m],
-                                                  PM ? keep_if(q[u][i][m], w[i], n - nb + u) : q[u][i][m]);
+                                            accum(acc[i * CPL + m], PM ? keep_if(q[i][m], w[i], n - nb + u) : q[i][m]);
+                                    __builtin_amdgcn_sched_barrier(0);
                                 }
+                            }
+                            for (; n < ne; n++) {
+                                const int o = cload(A.offs, n) * K;
+                                float q[RI][CPL];
+#pragma unroll
+                                for (int i = 0; i < RI; i++) lds_get<CPL>(smem + base[i] + o, q[i]);
+#pragma unroll
+                                for (int i = 0; i < RI; i++)
+#pragma unroll
+                                    for (int m = 0; m < CPL; m++)
+                                        accum(acc[i * CPL + m], PM ? keep_if(q[i][m], w[i], n - nb) : q[i][m]);
                             }
                         }
                         __syncthreads();
