@@ -186,7 +186,9 @@ typedef struct tc_matrix_stats {
                                held as cleared bits of a shape's term-presence
                                bitmap instead of in cols */
     int bitmap_shapes;      /* shapes with a term-presence bitmap */
-    int reserved;
+    int near_window;        /* 1: most remainder entries (24 or more a row) lie
+                               within 64 rows of their row, so the k > 1 tiles
+                               (k <= 8) read them from x rows staged in LDS */
 } tc_matrix_stats;
 
 /* upload a TaiChi_new (host) into the device layout. nnz is the count used

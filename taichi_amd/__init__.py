@@ -47,7 +47,7 @@ class MatrixStats(C.Structure):
                 ("num_negs", C.c_int), ("long_rows", C.c_int), ("offset_bytes", C.c_int),
                 ("nnz", C.c_longlong), ("remainder", C.c_longlong),
                 ("format_bytes", C.c_longlong), ("device_bytes", C.c_longlong),
-                ("holes", C.c_longlong), ("bitmap_shapes", C.c_int), ("reserved", C.c_int)]
+                ("holes", C.c_longlong), ("bitmap_shapes", C.c_int), ("near_window", C.c_int)]
 
 
 class Config(C.Structure):

@@ -234,6 +234,7 @@ int plan_layout(int M, int N, const TaiChi_new *T, int long_threshold, Layout &P
     std::mutex bm;
     tc::parallel_for(M, [&](long long b, long long e) {
         int lbad = 0;
+        long long lnear = 0;
         for (long long r = b; r < e; r++) {
             long long o = dst[r];
             for (int j = T->csrRow[r]; j < T->csrRow[r + 1]; j++) {
@@ -245,13 +246,13 @@ int plan_layout(int M, int N, const TaiChi_new *T, int long_threshold, Layout &P
                     P.cols[o++] = 1;
                     continue;
                 }
+                lnear += c >= r - tcd::kNearRows && c <= r + tcd::kNearRows;
                 P.cols[o++] = encoded ? raw : ((raw << 1) | 1);
             }
         }
-        if (lbad) {
-            std::lock_guard<std::mutex> g(bm);
-            bad = 1;
-        }
+        std::lock_guard<std::mutex> g(bm);
+        P.near += lnear;
+        bad |= lbad;
     });
     if (bad) return tc::fail(TC_ERR_INVALID, "matrix layout: column index out of range");
     return TC_OK;

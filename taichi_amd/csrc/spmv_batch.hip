@@ -168,7 +168,7 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     } else if (kern == 1 || kern == 3) {
         const int trw = wide_rows(K, kBatchRI, (int)sizeof(XT));
-        const size_t lds = std::max((size_t)(trw + kWideHalo) * K * 4, (size_t)trw * K * 8);
+        const size_t lds = wide_lds_bytes(K, trw, false, true);  // batched tiles run without the near window
         if (kern == 3) {
             if (K == 2) batch_wide_kernel<2, XT, true><<<grid, kWG, lds, s>>>(D, UB, nd, units);
             else if (K == 4) batch_wide_kernel<4, XT, true><<<grid, kWG, lds, s>>>(D, UB, nd, units);

@@ -120,6 +120,7 @@ struct Layout {
     int M = 0, N = 0, S = 0, D = 0, G = 0, L = 0, ow = 1;
     long long R = 0;      // entries kept in cols
     long long holes = 0;  // cancelled zero cells absorbed by presence bitmaps
+    long long near = 0;   // kept entries whose column is within kNearRows of their row
     std::vector<ShapeDev> shapes;
     std::vector<uint32_t> pres;
     std::vector<int> gbase, glong, lrow, lptr, cols;

@@ -53,6 +53,8 @@ namespace tcd {
 constexpr int kWG = 256;
 constexpr int kWaves = kWG / 64;
 constexpr int kStageU = 8;  // loads in flight per lane when staging streams
+constexpr int kNearRows = 64;  // row distance of a remainder entry counted as near its row
+constexpr int kNearMin = 24;   // near entries a row (on average) for the wide tiles' near window
 
 // Non-temporal stores of 1, 2 or 4 floats (y: written once, not read back by
 // the kernel; streaming it past L2 leaves the cache to the x lines gathers
@@ -86,6 +88,7 @@ struct MatView {
     int M, N, S, L;
     int ow;        // bytes per local row offset: 1 or 2
     int ngroups;
+    int near;      // 1: most remainder entries lie within kNearRows of their row (bands)
     ShapeDev sh0;  // copy of shapes[0]: arrives with the kernel arguments
     const ShapeDev *shapes;
     const int *offs;
@@ -838,14 +841,55 @@ __host__ __device__ constexpr int wide_cpl(int K, int esize) { return esize == 2
 // masked). A masked chunk runs k = 3, 5, 6, 7 as one pass of width 4 or 8
 // instead of 2 + 1, 4 + 1, 4 + 2, 4 + 2 + 1 passes. Needs kv > K/2 (so a
 // lane's vector stays inside its X row, or X's last row is handled apart).
-// LDS: the x window [(TRW + kWideHalo) * K] floats, reused for the long-row
-// sums (TRW * K doubles; the launch sizes LDS for the larger of the two).
+// LDS: the x window [(TRW + kWideHalo) * K] floats, reused for the rows
+// around the tile that a near remainder reads ((TRW + 2 kWideHalo) * K
+// floats, K <= 8) and for the long-row sums (TRW * K doubles); the launch
+// sizes LDS by wide_lds_bytes.
 constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
 
-template <int K, typename XT, int RI, bool LONG, bool MASK = false>
+// LDS bytes of a wide-rows tile of trw rows and K columns: the shape window
+// (or, near, the rows around the tile the remainder reads) and the long-row
+// sums (lng)
+__host__ __device__ constexpr size_t wide_lds_bytes(int K, int trw, bool near, bool lng)
+{
+    const size_t win = (size_t)(trw + (near && K <= 8 ? 2 : 1) * kWideHalo) * K * 4;
+    const size_t lsum = lng ? (size_t)trw * K * 8 : 0;
+    return win > lsum ? win : lsum;
+}
+
+// Stage x rows [w0, w0 + nrows), all K columns, in LDS as smem[(row - w0) * K
+// + col], CPL-wide vectors, four loads in flight per lane (masked chunks: the
+// columns at or past kv read as 0, as the lane's own loads would give them)
+template <int K, typename XT, int CPL, bool MASK>
+__device__ __forceinline__ void stage_x_rows(const XT *__restrict__ X, int ldx, int N, int w0, int nrows, int kv,
+                                             float *smem)
+{
+    constexpr int LPR = K / CPL;
+    const int nw = nrows * LPR;  // CPL-wide vectors of the window
+    const int tid = threadIdx.x;
+    for (int b = 0; b < nw; b += kWG * 4) {
+        float tv[4][CPL];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int v = min(b + u * kWG + tid, nw - 1);
+            const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
+            if constexpr (MASK)
+                load_qm<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
+            else load_q<XT, CPL>(xp, tv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int v = b + u * kWG + tid;
+            if (v < nw) lds_put<CPL>(smem + (v / LPR) * K + (v % LPR) * CPL, tv[u]);
+        }
+    }
+}
+
+template <int K, typename XT, int RI, bool LONG, bool MASK = false, bool NEAR = false>
 __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
                                           int ldy, int t, float *smem, int kv = K)
 {
+    static_assert(!NEAR || K <= 8, "the near window is for K <= 8");
     constexpr int CPL = wide_cpl(K, (int)sizeof(XT));
     constexpr int LPR = K / CPL;
     constexpr int GPW = kWG / LPR;  // row groups per workgroup
@@ -903,26 +947,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     if (PM) pres_words(A, sh, u0 + GPW * i + g, plo[i], phi[i]);
                 }
                 if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
-                    const int nw = (w1 - w0) * LPR;  // CPL-wide vectors of the window
-                    for (int b = 0; b < nw; b += kWG * 4) {
-                        float tv[4][CPL];
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const int v = min(b + u * kWG + tid, nw - 1);
-                            const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
-                            if constexpr (MASK)
-                                load_qm<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL),
-                                                 w0 + v / LPR == A.N - 1, tv[u]);
-                            else load_q<XT, CPL>(xp, tv[u]);
-                        }
-#pragma unroll
-                        for (int u = 0; u < 4; u++) {
-                            const int v = b + u * kWG + tid;
-                            if (v < nw) {
-                                lds_put<CPL>(smem + (v / LPR) * K + (v % LPR) * CPL, tv[u]);
-                            }
-                        }
-                    }
+                    stage_x_rows<K, XT, CPL, MASK>(X, ldx, A.N, w0, w1 - w0, kv, smem);
                     __syncthreads();
                     // every row of the wavefront inside the shape with all its
                     // columns: DU offsets per step, their scalar loads, then
@@ -1034,6 +1059,18 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             else shape_terms(std::false_type{});
         }
 
+        // NEAR (the matrix's A.near, K <= 8): its remainder entries mostly
+        // lie near their rows, so x rows [u0 - kWideHalo, u0 + TRW +
+        // kWideHalo) are staged in LDS once and an entry inside them reads x
+        // there, not through the texture path
+        int wl = 0, wn = 0;
+        if constexpr (NEAR) {
+            wl = max(0, u0 - kWideHalo);
+            wn = max(0, min(A.N, u0 + TRW + kWideHalo) - wl);
+            stage_x_rows<K, XT, CPL, MASK>(X, ldx, A.N, wl, wn, kv, smem);
+            __syncthreads();
+        }
+
         // remainder entries, U in flight, added in stored order; the next U
         // column indices load before this step's gathers, so a long row pays
         // one memory trip per step, not two
@@ -1053,9 +1090,10 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
 #pragma unroll
                 for (int u = 0; u < U; u++)
                     if (j0 + u < re[i]) {
-                        if constexpr (MASK)
-                            load_qm<XT, CPL>(X + (size_t)(v[u] >> 1) * ldx + c0, nv, (v[u] >> 1) == A.N - 1, xv[u]);
-                        else load_q<XT, CPL>(X + (size_t)(v[u] >> 1) * ldx + c0, xv[u]);
+                        const int c = v[u] >> 1;
+                        if (NEAR && (unsigned)(c - wl) < (unsigned)wn) lds_get<CPL>(smem + (c - wl) * K + c0, xv[u]);
+                        else if constexpr (MASK) load_qm<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xv[u]);
+                        else load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[u]);
                     }
 #pragma unroll
                 for (int u = 0; u < U; u++)
@@ -1065,6 +1103,8 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                             accum(acc[i * CPL + m], (v[u] & 1) ? xv[u][m] : -xv[u][m]);
             }
         }
+
+        if constexpr (NEAR) __syncthreads();  // the window is no longer read
 
         if constexpr (LONG)
             long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, (double *)smem, TRW * K,

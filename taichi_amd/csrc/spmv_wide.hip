@@ -2,19 +2,23 @@
 // row segments) and its launcher.
 #include "spmv_host.h"
 
+#include <type_traits>
+
 namespace tcd {
 
-template <int K, typename XT, int RI, bool LONG, bool MASK>
+template <int K, typename XT, int RI, bool LONG, bool MASK, bool NEAR>
 __global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
                                                         float *__restrict__ Y, int ldy, int ntiles, int kv)
 {
     extern __shared__ float smem[];
     int tb, te;
     block_units(ntiles, &tb, &te);
-    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG, MASK>(A, X, ldx, Y, ldy, t, smem, kv);
+    for (int t = tb; t < te; t++) wide_tile<K, XT, RI, LONG, MASK, NEAR>(A, X, ldx, Y, ldy, t, smem, kv);
 }
 
-// kv < K: a masked chunk (only K = 2, 4, 8 are instantiated masked)
+// kv < K: a masked chunk (only K = 2, 4, 8 are instantiated masked). A
+// matrix whose remainder lies near its rows (A.near) runs the near-window
+// instantiation at K <= 8; every other matrix runs the kernel without it.
 template <int K, typename XT, int RI>
 int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, int grid, int kv, hipStream_t s)
 {
@@ -23,20 +27,28 @@ int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
     // the x window (floats), reused for the long-row sums (doubles)
-    const size_t lds = (A.S > 0 || A.L > 0)
-                           ? std::max((size_t)(TRW + kWideHalo) * K * 4, A.L > 0 ? (size_t)TRW * K * 8 : 0)
-                           : 0;
+    const bool near = K <= 8 && A.near;
+    const size_t lds = (A.S > 0 || A.L > 0 || near) ? wide_lds_bytes(K, TRW, near, A.L > 0) : 0;
     const XT *x = (const XT *)X;
+    auto go = [&](auto lng, auto msk, auto nr) {
+        spmv_wide_kernel<K, XT, RI, decltype(lng)::value, decltype(msk)::value, decltype(nr)::value>
+            <<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, decltype(msk)::value ? kv : K);
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
     if constexpr (K <= 8) {
-        if (kv < K) {
-            if (A.L > 0) spmv_wide_kernel<K, XT, RI, true, true><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, kv);
-            else spmv_wide_kernel<K, XT, RI, false, true><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, kv);
-            HIPCHECK(hipGetLastError());
-            return TC_OK;
+        const bool m = kv < K;
+        if (near) {
+            if (A.L > 0) m ? go(T_{}, T_{}, T_{}) : go(T_{}, F_{}, T_{});
+            else m ? go(F_{}, T_{}, T_{}) : go(F_{}, F_{}, T_{});
+        } else {
+            if (A.L > 0) m ? go(T_{}, T_{}, F_{}) : go(T_{}, F_{}, F_{});
+            else m ? go(F_{}, T_{}, F_{}) : go(F_{}, F_{}, F_{});
         }
+    } else {
+        if (A.L > 0) go(T_{}, F_{}, F_{});
+        else go(F_{}, F_{}, F_{});
     }
-    if (A.L > 0) spmv_wide_kernel<K, XT, RI, true, false><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, K);
-    else spmv_wide_kernel<K, XT, RI, false, false><<<G, kWG, lds, s>>>(A, x, ldx, Y, ldy, ntiles, K);
     HIPCHECK(hipGetLastError());
     return TC_OK;
 }
