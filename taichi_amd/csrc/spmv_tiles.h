@@ -169,6 +169,9 @@ __device__ __forceinline__ float keep_if(float v, uint32_t w, int j)
     return __int_as_float(__float_as_int(v) & __builtin_amdgcn_sbfe((int)w, j, 1));
 }
 
+// LDS index idx if bit j of w is set, else 0 (where a run of +0.0 is kept)
+__device__ __forceinline__ int keep_idx(int idx, uint32_t w, int j) { return idx & __builtin_amdgcn_sbfe((int)w, j, 1); }
+
 // start of row i's short entries (i in [0, M]); row i ends at row_start(i+1)
 __device__ __forceinline__ int row_start(const MatView &A, int i)
 {
@@ -846,13 +849,14 @@ __host__ __device__ constexpr int wide_cpl(int K, int esize) { return esize == 2
 // floats, K <= 8) and for the long-row sums (TRW * K doubles); the launch
 // sizes LDS by wide_lds_bytes.
 constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
+constexpr int kWideZero = 8;   // floats of +0.0 before a shape's x window (absent terms read them)
 
 // LDS bytes of a wide-rows tile of trw rows and K columns: the shape window
 // (or, near, the rows around the tile the remainder reads) and the long-row
 // sums (lng)
 __host__ __device__ constexpr size_t wide_lds_bytes(int K, int trw, bool near, bool lng)
 {
-    const size_t win = (size_t)(trw + (near && K <= 8 ? 2 : 1) * kWideHalo) * K * 4;
+    const size_t win = ((size_t)(trw + (near && K <= 8 ? 2 : 1) * kWideHalo) * K + kWideZero) * 4;
     const size_t lsum = lng ? (size_t)trw * K * 8 : 0;
     return win > lsum ? win : lsum;
 }
@@ -947,7 +951,12 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     if (PM) pres_words(A, sh, u0 + GPW * i + g, plo[i], phi[i]);
                 }
                 if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
-                    stage_x_rows<K, XT, CPL, MASK>(X, ldx, A.N, w0, w1 - w0, kv, smem);
+                    // the window starts past kWideZero floats of +0.0: a term
+                    // a bitmap marks absent reads them (its LDS index masked
+                    // to 0), so one AND per row and offset replaces one per
+                    // column
+                    stage_x_rows<K, XT, CPL, MASK>(X, ldx, A.N, w0, w1 - w0, kv, smem + kWideZero);
+                    if (tid < kWideZero) smem[tid] = 0.0f;
                     __syncthreads();
                     // every row of the wavefront inside the shape with all its
                     // columns: DU offsets per step, their scalar loads, then
@@ -964,7 +973,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                         const int nseg = PM ? (sh.oe - sh.ob + 31) >> 5 : 1;
                         int base[RI];
 #pragma unroll
-                        for (int i = 0; i < RI; i++) base[i] = (u0 + GPW * i + g + shift - w0) * K + c0;
+                        for (int i = 0; i < RI; i++) base[i] = kWideZero + (u0 + GPW * i + g + shift - w0) * K + c0;
                         for (int seg = 0; seg < nseg; seg++) {
                             const int nb = sh.ob + 32 * seg, ne = PM ? min(sh.oe, nb + 32) : sh.oe;
                             uint32_t w[RI];
@@ -986,12 +995,14 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                                 for (int u = 0; u < DU; u++) {
                                     float q[RI][CPL];
 #pragma unroll
-                                    for (int i = 0; i < RI; i++) lds_get<CPL>(smem + base[i] + o[u], q[i]);
+                                    for (int i = 0; i < RI; i++)
+                                        lds_get<CPL>(smem + (PM ? keep_idx(base[i] + o[u], w[i], n - nb + u)
+                                                                : base[i] + o[u]),
+                                                     q[i]);
 #pragma unroll
                                     for (int i = 0; i < RI; i++)
 #pragma unroll
-                                        for (int m = 0; m < CPL; m++)
-                                            accum(acc[i * CPL + m], PM ? keep_if(q[i][m], w[i], n - nb + u) : q[i][m]);
+                                        for (int m = 0; m < CPL; m++) accum(acc[i * CPL + m], q[i][m]);
                                     __builtin_amdgcn_sched_barrier(0);
                                 }
                             }
@@ -999,12 +1010,12 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                                 const int o = cload(A.offs, n) * K;
                                 float q[RI][CPL];
 #pragma unroll
-                                for (int i = 0; i < RI; i++) lds_get<CPL>(smem + base[i] + o, q[i]);
+                                for (int i = 0; i < RI; i++)
+                                    lds_get<CPL>(smem + (PM ? keep_idx(base[i] + o, w[i], n - nb) : base[i] + o), q[i]);
 #pragma unroll
                                 for (int i = 0; i < RI; i++)
 #pragma unroll
-                                    for (int m = 0; m < CPL; m++)
-                                        accum(acc[i * CPL + m], PM ? keep_if(q[i][m], w[i], n - nb) : q[i][m]);
+                                    for (int m = 0; m < CPL; m++) accum(acc[i * CPL + m], q[i][m]);
                             }
                         }
                         __syncthreads();
@@ -1020,7 +1031,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                                 const int j = n - sh.ob;
                                 const uint32_t w = j < 32 ? plo[i] : phi[i];
                                 float q[CPL];
-                                lds_get<CPL>(smem + (c - w0) * K + c0, q);
+                                lds_get<CPL>(smem + kWideZero + (c - w0) * K + c0, q);
 #pragma unroll
                                 for (int m = 0; m < CPL; m++)
                                     accum(acc[i * CPL + m], PM ? keep_if(q[m], w, j & 31) : q[m]);
