@@ -35,16 +35,16 @@ int cu_count()
 // diagonal terms together average more than 24 a row (wide bands), use
 // 256-row tiles at k = 1: a tile's entries pass through the stage in about
 // two rounds, and the tiles are short enough to spread the band's ends.
-// Wide bands also take two rows per lane group at k <= 8 (the window
-// staged once serves twice the rows): 57 diagonals of 923k rows, k = 8
-// measured 62 us against 72 us with one.
+// Two rows per lane group (a config choice) measured up to 17 % slower than
+// one on bands of 4 to 57 diagonals at k = 4..8 (57 diagonals of 923k rows,
+// BF16 k = 6: 65 against 54 us), 2-3 % faster only for FP32 k = 6 on two.
 tc_config default_config(int K, int esize, long long remainder, long long diag_terms, int M)
 {
     tc_config c{};
     const bool band = M > 0 && remainder + diag_terms > 24ll * M;
     const bool heavy = M > 0 && (remainder > 8ll * M || band);
     c.rows_per_tile = K == 1      ? (heavy ? 256 : 512)
-                      : wide_k(K) ? wide_rows(K, band && K <= 8 ? 2 : 1, esize)
+                      : wide_k(K) ? wide_rows(K, 1, esize)
                                   : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
