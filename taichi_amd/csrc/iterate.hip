@@ -91,7 +91,7 @@ struct PrEnd {
     float base, alpha, spread;
     int write_r;
     float &d;
-    __device__ void operator()(int i, double sum) const
+    __device__ void operator()(int i, tcd::acc_t sum) const
     {
         const float ri = base + alpha * ((float)sum + spread);
         const float w = inv_deg[i];

@@ -7,10 +7,13 @@
 // group bases plus 1- or 2-byte local offsets, rows longer than the long-row
 // threshold move to a table of their own, and the sign-0 entries that cancel
 // zero cells of a shape's dense diagonals (TaiChi.cu:500-520 writes them)
-// become cleared bits of a per-row presence bitmap when that is smaller:
-// a shape gets a bitmap (ceil(D_s/32) words a row) when more of its zero
-// cells are cancelled than it has bitmap words. The values y = A x stay the
-// same; only the terms a kernel reads change.
+// become cleared bits of a per-row presence bitmap. A shape gets a bitmap
+// (ceil(D_s/32) words a row) when more of its zero cells are cancelled than
+// it has bitmap words, and also whenever one of its cancellations would be
+// inexact in FP32 accumulation: that is every zero cell except one that is
+// its row's only diagonal term on a row summed by one lane (there the term
+// comes first and its sign-0 entry second, and x - x is exactly +0.0). The
+// values y = A x stay the same; only the terms a kernel reads change.
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -111,30 +114,55 @@ int plan_layout(int M, int N, const TaiChi_new *T, int long_threshold, Layout &P
             std::sort(c.offs.begin(), c.offs.end());
             cand.push_back(std::move(c));
         }
+    // diagonal terms a kernel adds for row r (the columns of every shape's
+    // offsets that fall inside its square), counted up to 2
+    auto diag_terms_of = [&](int r) {
+        int n = 0;
+        for (int s = 0; s < S && n < 2; s++) {
+            const ShapeDev &h = P.shapes[s];
+            if (r < h.xs || r >= h.xe) continue;
+            const int side = h.xe - h.xs;
+            for (int k = h.ob; k < h.oe && n < 2; k++) {
+                const long long c = (long long)r - h.xs + h.ys + T->neg_offsets[k];
+                n += c >= h.ys && c < (long long)h.ys + side;
+            }
+        }
+        return n;
+    };
     std::vector<ShapeOffs> pm_shapes;  // candidates that get a bitmap, in shape order
     if (!cand.empty()) {
         std::vector<long long> cnt(cand.size(), 0);
+        std::vector<char> need(cand.size(), 0);  // a cancellation of the shape is inexact in FP32
         std::mutex mu;
         tc::parallel_for(M, [&](long long b, long long e) {
             std::vector<long long> lc(cand.size(), 0);
-            for (long long r = b; r < e; r++)
+            std::vector<char> ln(cand.size(), 0);
+            for (long long r = b; r < e; r++) {
+                int inexact = -1;  // computed at the row's first sign-0 entry
                 for (int j = T->csrRow[r]; j < T->csrRow[r + 1]; j++) {
                     const int raw = T->csrCol[j];
                     if (raw < 0 || (raw & 1)) continue;
                     for (size_t ci = 0; ci < cand.size(); ci++)
                         if (match_term(P.shapes[cand[ci].s], cand[ci], (int)r, raw >> 1, nullptr) >= 0) {
                             lc[ci]++;
+                            if (inexact < 0)
+                                inexact = T->csrRow[r + 1] - T->csrRow[r] > LT || diag_terms_of((int)r) >= 2;
+                            ln[ci] |= (char)inexact;
                             break;
                         }
                 }
+            }
             std::lock_guard<std::mutex> g(mu);
-            for (size_t ci = 0; ci < cand.size(); ci++) cnt[ci] += lc[ci];
+            for (size_t ci = 0; ci < cand.size(); ci++) {
+                cnt[ci] += lc[ci];
+                need[ci] |= ln[ci];
+            }
         });
         long long words = 0;
         for (size_t ci = 0; ci < cand.size(); ci++) {
             ShapeDev &h = P.shapes[cand[ci].s];
             const long long w = (long long)(h.xe - h.xs) * (((h.oe - h.ob) + 31) / 32);
-            if (cnt[ci] <= w || words + w > (1ll << 30)) continue;
+            if ((cnt[ci] <= w && !need[ci]) || words + w > (1ll << 30)) continue;
             h.pm = (int)words;
             words += w;
             pm_shapes.push_back(cand[ci]);

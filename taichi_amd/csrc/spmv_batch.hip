@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restr
     extern __shared__ float smem[];
     __shared__ int s_ub[kBatchLdsDesc];
     float *s_val = smem;
-    double *s_lsum = (double *)(s_val + lsum_at(capv));
+    acc_t *s_lsum = (acc_t *)(s_val + lsum_at(capv));
     int *s_start = (int *)(s_lsum + kWG * kBatchP);
     batch_loop(D, UB, nd, total, s_ub, [&](const WorkDesc &w, int t) {
         run_tile<K, XT, kBatchP>(w.A, (const XT *)w.X, w.ldx, w.Y, w.ldy, t * TRW, s_val, capv, s_start, s_lsum);
@@ -164,7 +164,7 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
     const int grid = units;
     if (kern == 0) {
         constexpr int TRW = kWG * kBatchRPT;
-        const size_t lds = ((size_t)lsum_at(kBatchCapv) + 3 * TRW) * 4;  // s_val | long sums (doubles) | window
+        const size_t lds = ((size_t)lsum_at(kBatchCapv) + 3 * TRW) * 4;  // s_val | long sums (2*TRW floats reserved) | window
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     } else if (kern == 1 || kern == 3) {
         const int trw = wide_rows(K, kBatchRI, (int)sizeof(XT));

@@ -15,7 +15,7 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
     int tb, te;
     block_units(ntiles, &tb, &te);
     for (int t = tb; t < te; t++)
-        rows_tile<XT, RPT, OW, FAST, LONG>(A, X, STR ? ldx : 1, Y, STR ? ldy : 1, t, smem, capv);
+        rows_tile<XT, RPT, OW, FAST, LONG>(A, X, STR ? ldx : 1, Y, STR ? ldy : 1, t, smem, capv, StoreY{}, t + 1 < te);
 }
 
 // ---- single right-hand side, quad rows: at most one shape with one offset ----
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kWG) void spmv_rows_kernel(MatView A, const XT *__r
 // (texture addresser busy most of the kernel), not by bytes. Same canonical
 // per-row order as every other kernel. Requires X and Y aligned to Q
 // elements (checked on the host).
-// LDS: s_val[capv] (also the long-row sums afterwards: TRW doubles, so capv
+// LDS: s_val[capv] (also the long-row sums afterwards: TRW floats; capv
 // >= 2*TRW when the matrix has long rows) | s_win[TRW] (DIAG).
 template <int OW, int Q>
 __device__ __forceinline__ void load_ends(const void *roff, int r0, int (&e)[Q])
@@ -101,7 +101,7 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
     int4 cv = make_int4(1, 1, 1, 1);
     if (4 * tid < min(EA, capv)) cv = *(const int4 *)(A.cols + lb + 4 * tid);
 
-    double acc[Q];
+    acc_t acc[Q];
 #pragma unroll
     for (int i = 0; i < Q; i++) acc[i] = 0.0;
     if constexpr (DIAG) {
@@ -153,7 +153,7 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
 #pragma unroll
         for (int i = 0; i < Q; i++) {
             const int lo = max(rs[i], e0), hi = min(re[i], e0 + ne);
-            double a = acc[i];
+            acc_t a = acc[i];
             sum_run<1>(a, s_val, lo - e0, hi - e0);
             acc[i] = a;
         }
@@ -165,7 +165,7 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
         bool pv[Q];
 #pragma unroll
         for (int i = 0; i < Q; i++) prow[i] = Q * tid + i, lq[i] = prow[i], pv[i] = r0 + i < A.M;
-        long_phase<1, XT, Q>(A, X, 1, u0, min(TRW, A.M - u0), lq, pv, acc, (double *)s_val, TRW);
+        long_phase<1, XT, Q>(A, X, 1, u0, min(TRW, A.M - u0), lq, pv, acc, (acc_t *)s_val, TRW);
     }
 
     if (full) {
@@ -294,7 +294,7 @@ bool quad_applies(const MatView &A, const void *X, const float *Y, int esize, co
     if (!(A.S == 0 || (A.S == 1 && A.sh0.oe - A.sh0.ob == 1 && A.sh0.pm < 0))) return false;
     const int Q = c.rows_per_tile == 512 ? 2 : c.rows_per_tile == 1024 ? 4 : 0;
     if (Q == 0 || capv % 4 != 0 || capv < kWG * Q) return false;
-    if (A.L > 0 && capv < 2 * kWG * Q) return false;  // the long-row sums (doubles) reuse s_val
+    if (A.L > 0 && capv < 2 * kWG * Q) return false;  // the long-row sums reuse s_val
     return ((uintptr_t)X % ((uintptr_t)Q * esize)) == 0 && ((uintptr_t)Y % ((uintptr_t)Q * 4)) == 0;
 }
 

@@ -12,7 +12,7 @@ __global__ __launch_bounds__(kWG) void spmv_kernel(MatView A, const XT *__restri
     extern __shared__ float smem[];
     constexpr int TRW = (kWG * P) / K;
     float *s_val = smem;
-    double *s_lsum = (double *)(s_val + lsum_at(capv));
+    acc_t *s_lsum = (acc_t *)(s_val + lsum_at(capv));
     int *s_start = (int *)(s_lsum + kWG * P);
     int ub, ue;
     block_units(nunits, &ub, &ue);

@@ -35,7 +35,7 @@
 //      gathered and signed, staged in LDS, summed by one lane per row);
 //   3. long rows: one wavefront each, 64-lane strided partial sums and a
 //      fixed DPP/ds_swizzle butterfly.
-// Every term is added in FP64 (accum) and y is rounded to FP32 once.
+// Every term is added in FP32 (accum) into a sum that starts at +0.0.
 // The order never depends on the tile size, the grid, the column chunking or
 // the batch a request runs in, so a request gives bit-identical y alone, in a
 // batch, or with another configuration.
@@ -73,7 +73,7 @@ __device__ __forceinline__ void st_nt4(float *p, float a, float b, float c, floa
     __builtin_nontemporal_store(v, (f32x4 *)p);
 }
 
-// Offset (in floats) of a double array placed after capv floats of LDS
+// Offset (in floats) of an 8-byte-aligned array placed after capv floats of LDS
 __host__ __device__ constexpr int lsum_at(int capv) { return (capv + 1) & ~1; }
 
 // One shape: rows [xs, xe), offsets offs[ob..oe) (spread omin..omax), term
@@ -161,7 +161,7 @@ __device__ __forceinline__ void pres_words(const MatView &A, const ShapeDev &sh,
 }
 
 // v if bit j of w is set, else +0.0, without a branch: the signed one-bit
-// field is 0 or all ones. Adding +0.0 leaves an FP64 sum that started at +0.0
+// field is 0 or all ones. Adding +0.0 leaves a sum that started at +0.0
 // unchanged (it can never be -0.0), so skipping a term and adding this zero
 // give the same bits.
 __device__ __forceinline__ float keep_if(float v, uint32_t w, int j)
@@ -180,47 +180,41 @@ __device__ __forceinline__ int row_start(const MatView &A, int i)
     return b;
 }
 
-// Accumulation: every term (an FP32 value) is converted to FP64 and added
-// to a double in the canonical order; y is that double rounded to FP32 once.
-// A sum of n FP32 terms in FP64 is within n * 2^-53 * sum|terms| of the
-// exact sum, so the x values a dense diagonal adds at its zero cells and the
-// remainder subtracts again cancel to far below one FP32 ulp of the row's
-// true terms, and y is the FP32 rounding of a nearly exact sum. Integer-valued
-// terms stay exact. On gfx950 a convert plus a v_add_f64 costs less than a
-// compensated FP32 TwoSum (measured 16.7 vs 9.3 G terms/s).
-__device__ __forceinline__ void accum(double &s, float x) { s += (double)x; }
+// Accumulation: FP32, as the reference's kernels and CPU check sum
+// (TaiChi.cu:1004-1024, main.cu:61-69). Every (row, column) pair adds its
+// terms in the canonical order into a float that starts at +0.0, and y is
+// that float. An FP32 sum of n terms of one sign-free kind is within
+// (n-1) * 2^-24 * sum|terms| of the exact sum. What could spoil that is
+// cancellation: a dense diagonal adds x over a whole segment and a sign-0
+// remainder entry subtracts it again at a zero cell. The device layout
+// leaves no inexact cancellation: a zero cell of a row whose only diagonal
+// term is that cell cancels exactly (the term is the row's first, the
+// sign-0 entry its second: x - x = +0.0), and every other zero cell is a
+// cleared bit of a presence bitmap instead of a sign-0 entry (layout.hip).
+// Integer-valued terms stay exact (below 2^24), so y is bit-identical to the
+// reference's CPU sums for integer x.
+typedef float acc_t;
+__device__ __forceinline__ void accum(acc_t &s, float x) { s += x; }
 
 template <int CTRL>
-__device__ __forceinline__ double dpp_d(double v)
+__device__ __forceinline__ float dpp_f(float v)
 {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// Sum over the 64 lanes of a wavefront in a fixed order, in FP64: quad swaps
-// and row mirrors by DPP, the 16-lane swap by ds_swizzle, halves by readlane;
-// each 64-bit value moves as two 32-bit halves. Every lane ends with the same
-// value (each pairwise add sees the same two operands).
-__device__ __forceinline__ double wave_sum_d(double v)
+// Sum over the 64 lanes of a wavefront in a fixed order: quad swaps and row
+// mirrors by DPP, the 16-lane swap by ds_swizzle, halves by readlane. Every
+// lane ends with the same value (each pairwise add sees the same two
+// operands).
+__device__ __forceinline__ float wave_sum(float v)
 {
-    v += dpp_d<0xB1>(v);   // quad_perm 1,0,3,2
-    v += dpp_d<0x4E>(v);   // quad_perm 2,3,0,1
-    v += dpp_d<0x141>(v);  // row_half_mirror
-    v += dpp_d<0x140>(v);  // row_mirror
-    {
-        const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_ds_swizzle((int)b, 0x401F);  // lane ^ 16
-        const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
-        v += __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-    }
-    const long long b = __double_as_longlong(v);
-    const long long l0 = ((long long)__builtin_amdgcn_readlane((int)(b >> 32), 0) << 32) |
-                         (unsigned)__builtin_amdgcn_readlane((int)b, 0);
-    const long long l32 = ((long long)__builtin_amdgcn_readlane((int)(b >> 32), 32) << 32) |
-                          (unsigned)__builtin_amdgcn_readlane((int)b, 32);
-    return __longlong_as_double(l0) + __longlong_as_double(l32);
+    v += dpp_f<0xB1>(v);   // quad_perm 1,0,3,2
+    v += dpp_f<0x4E>(v);   // quad_perm 2,3,0,1
+    v += dpp_f<0x141>(v);  // row_half_mirror
+    v += dpp_f<0x140>(v);  // row_mirror
+    v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));  // lane ^ 16
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0)) +
+           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
 }
 
 // Bijection of block ids so that each XCD (blocks are dispatched round-robin
@@ -251,7 +245,7 @@ __device__ __forceinline__ void block_units(int total, int *b, int *e)
 // acc += s[i * K] for i in [lo, hi), in order, with four LDS reads in flight
 // (a lane's serial sum of its row's staged entries)
 template <int K>
-__device__ __forceinline__ void sum_run(double &acc, const float *s, int lo, int hi)
+__device__ __forceinline__ void sum_run(acc_t &acc, const float *s, int lo, int hi)
 {
     int i = lo;
     for (; i + 4 <= hi; i += 4) {
@@ -307,7 +301,7 @@ __device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne
 template <int K, typename XT, int P, bool PM>
 __device__ __forceinline__ void diag_one_shape_t(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
                                                const ShapeDev &sh, int ra, int rb, const int (&prow)[P],
-                                               const int (&pcol)[P], const bool (&pv)[P], double (&acc)[P],
+                                               const int (&pcol)[P], const bool (&pv)[P], acc_t (&acc)[P],
                                                float *s_val, int capv)
 {
     const int tid = threadIdx.x;
@@ -426,7 +420,7 @@ __device__ __forceinline__ void diag_one_shape_t(const MatView &A, const XT *__r
 template <int K, typename XT, int P>
 __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int s,
                                                const ShapeDev &sh, int ra, int rb, const int (&prow)[P],
-                                               const int (&pcol)[P], const bool (&pv)[P], double (&acc)[P],
+                                               const int (&pcol)[P], const bool (&pv)[P], acc_t (&acc)[P],
                                                float *s_val, int capv)
 {
     if (sh.pm >= 0)
@@ -437,7 +431,7 @@ __device__ __forceinline__ void diag_one_shape(const MatView &A, const XT *__res
 template <int K, typename XT, int P>
 __device__ __forceinline__ void diag_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
                                            const int (&prow)[P], const int (&pcol)[P], const bool (&pv)[P],
-                                           double (&acc)[P], float *s_val, int capv)
+                                           acc_t (&acc)[P], float *s_val, int capv)
 {
     for (int s = 0; s < A.S; s++) {
         const ShapeDev sh = s == 0 ? A.sh0 : load_shape(A, s);
@@ -447,21 +441,21 @@ __device__ __forceinline__ void diag_phase(const MatView &A, const XT *__restric
     }
 }
 
-// Long rows of the tile: one wavefront each, FP64 partial sums strided over
+// Long rows of the tile: one wavefront each, FP32 partial sums strided over
 // the 64 lanes and reduced by wave_sum_d; results land in s_lsum[local_row*K
-// + c] (doubles, 8-byte aligned) and are added to acc. Only columns c < kc
+// + c] and are added to acc. Only columns c < kc
 // are read (kc < K for a masked chunk; the others stay 0). Uniform control
 // flow; leaves s_lsum free.
 template <int K, typename XT, int P>
 __device__ __forceinline__ void long_phase(const MatView &A, const XT *__restrict__ X, int ldx, int u0, int rows,
-                                           const int (&lq)[P], const bool (&pv)[P], double (&acc)[P],
-                                           double *s_lsum, int nlsum, int kc = K)
+                                           const int (&lq)[P], const bool (&pv)[P], acc_t (&acc)[P],
+                                           acc_t *s_lsum, int nlsum, int kc = K)
 {
     if (A.L == 0) return;
     const int lb = cload(A.glong, u0 >> 6), le = cload(A.glong, ((u0 + rows - 1) >> 6) + 1);
     if (lb >= le) return;
     const int tid = threadIdx.x;
-    for (int q = tid; q < nlsum; q += kWG) s_lsum[q] = 0.0;
+    for (int q = tid; q < nlsum; q += kWG) s_lsum[q] = 0.0f;
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
     for (int l = lb + wave; l < le; l += kWaves) {
@@ -469,13 +463,13 @@ __device__ __forceinline__ void long_phase(const MatView &A, const XT *__restric
         if (row < u0 || row >= u0 + rows) continue;
         const int j0 = cload(A.lptr, l), j1 = cload(A.lptr, l + 1);
         for (int c = 0; c < kc; c++) {
-            double part = 0.0;
+            acc_t part = 0.0f;
             for (int j = j0 + lane; j < j1; j += 64) {
                 const int v = A.cols[j];
                 const float xv = load_x<XT>(X, (size_t)(v >> 1) * ldx + c);
                 accum(part, (v & 1) ? xv : -xv);
             }
-            part = wave_sum_d(part);
+            part = wave_sum(part);
             if (lane == 0) s_lsum[(row - u0) * K + c] = part;
         }
     }
@@ -487,13 +481,13 @@ __device__ __forceinline__ void long_phase(const MatView &A, const XT *__restric
 }
 
 // Canonical per-(row, column) order, shared by every kernel below, with
-// FP64 accumulation (accum) into acc and y = (float)acc:
+// FP32 accumulation (accum) into acc and y = acc:
 //   diagonal terms (shape order, offset order);
 //   short-row entries in stored order, one lane, serially;
 //   the long-row wavefront sum (long rows have no short entries).
 template <int K, typename XT, int P>
 __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
-                                         int ldy, int u0, float *s_val, int capv, int *s_start, double *s_lsum)
+                                         int ldy, int u0, float *s_val, int capv, int *s_start, acc_t *s_lsum)
 {
     constexpr int TRW = (kWG * P) / K;
     constexpr int NP = TRW * K;
@@ -503,7 +497,7 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
     // 1. row extents
     for (int r = tid; r <= TRW; r += kWG) s_start[r] = row_start(A, min(u0 + r, A.M));
 
-    double acc[P];
+    acc_t acc[P];
     int prow[P], pcol[P], lq[P];
     bool pv[P];
 #pragma unroll
@@ -533,7 +527,7 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
             if (!pv[p]) continue;
             const int lo = max(s_start[prow[p]] - ebase, e0);
             const int hi = min(s_start[prow[p] + 1] - ebase, e0 + ne);
-            double a = acc[p];
+            acc_t a = acc[p];
             sum_run<K>(a, s_val + pcol[p], lo - e0, hi - e0);
             acc[p] = a;
         }
@@ -558,26 +552,29 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
 // two scalar loads (tiles are 64-row aligned). All loads of a phase are
 // issued before the first use. Same canonical order as run_tile. Column c of
 // X is X[c * ldx], row r of Y is Y[r * ldy] (1 and 1 for a plain vector).
-// LDS: s_val[capv] | s_lsum[TRW] doubles (LONG) | s_win[TRW] (FAST).
+// LDS: s_val[capv] | s_lsum[TRW] (LONG; 2*TRW floats reserved) | s_win[TRW] (FAST).
 // PIPE (general path only): prefetch the next batch's column indices and sum
 // rows with four LDS reads in flight; the batched kernel, whose registers
 // serve every matrix kind in one launch, leaves it off.
 // The default end of a rows tile: y[row] = (float)sum.
 struct StoreY {
     static constexpr bool kStore = true;
-    __device__ void operator()(int, double) const {}
+    __device__ void operator()(int, acc_t) const {}
 };
 
-// EPI: what a finished row's FP64 sum becomes; StoreY writes it to Y, any
+// EPI: what a finished row's sum becomes; StoreY writes it to Y, any
 // other functor is called as epi(row, sum) instead (the iterative driver
 // turns it into the next iterate there).
+// more: another tile follows in this workgroup (its LDS writes then wait for
+// this tile's last reads); without one the tile's last barrier is skipped.
 template <typename XT, int RPT, int OW, bool FAST, bool LONG, bool PIPE = true, class EPI = StoreY>
 __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
-                                          int ldy, int t, float *smem, int capv, const EPI &epi = EPI{})
+                                          int ldy, int t, float *smem, int capv, const EPI &epi = EPI{},
+                                          bool more = true)
 {
     constexpr int TRW = kWG * RPT;
     float *s_val = smem;
-    double *s_lsum = (double *)(smem + lsum_at(capv));
+    acc_t *s_lsum = smem + lsum_at(capv);
     float *s_win = smem + lsum_at(capv) + (LONG ? 2 * TRW : 0);
     const int tid = threadIdx.x, lane = tid & 63;
     const uint8_t *ro8 = (const uint8_t *)A.roff;
@@ -593,7 +590,7 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
         constexpr int U = 4;
         int prow[RPT], pcol[RPT], lq[RPT], endv[RPT], gbv[RPT];
         bool pv[RPT];
-        double acc[RPT];
+        acc_t acc[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; j++) {
             prow[j] = j * kWG + tid;
@@ -701,7 +698,7 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
 #pragma unroll
             for (int j = 0; j < RPT; j++) {
                 const int lo = max(rs[j], e0), hi = min(re[j], e0 + ne);
-                double a = acc[j];
+                acc_t a = acc[j];
                 if constexpr (FAST || !PIPE) {  // rows of a few entries: the plain loop
                     for (int k = lo; k < hi; k++) accum(a, s_val[k - e0]);
                 } else {
@@ -709,7 +706,8 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
                 }
                 acc[j] = a;
             }
-            __syncthreads();
+            // long_phase writes s_lsum only, apart from s_val and s_win
+            if (more || e0 + capv < E) __syncthreads();
         }
 
         if constexpr (LONG) long_phase<1, XT, RPT>(A, X, ldx, u0, rows, lq, pv, acc, s_lsum, TRW);
@@ -846,7 +844,7 @@ __host__ __device__ constexpr int wide_cpl(int K, int esize) { return esize == 2
 // lane's vector stays inside its X row, or X's last row is handled apart).
 // LDS: the x window [(TRW + kWideHalo) * K] floats, reused for the rows
 // around the tile that a near remainder reads ((TRW + 2 kWideHalo) * K
-// floats, K <= 8) and for the long-row sums (TRW * K doubles); the launch
+// floats, K <= 8) and for the long-row sums (TRW * K floats); the launch
 // sizes LDS by wide_lds_bytes.
 constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
 constexpr int kWideZero = 8;   // floats of +0.0 before a shape's x window (absent terms read them)
@@ -857,7 +855,7 @@ constexpr int kWideZero = 8;   // floats of +0.0 before a shape's x window (abse
 __host__ __device__ constexpr size_t wide_lds_bytes(int K, int trw, bool near, bool lng)
 {
     const size_t win = ((size_t)(trw + (near && K <= 8 ? 2 : 1) * kWideHalo) * K + kWideZero) * 4;
-    const size_t lsum = lng ? (size_t)trw * K * 8 : 0;
+    const size_t lsum = lng ? (size_t)trw * K * sizeof(acc_t) : 0;
     return win > lsum ? win : lsum;
 }
 
@@ -907,7 +905,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
     const uint16_t *ro16 = (const uint16_t *)A.roff;
     {
         const int u0 = t * TRW;
-        double acc[P];
+        acc_t acc[P];
         int lq[P];
         bool pv[P];
         int rs[RI], re[RI];
@@ -1118,7 +1116,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
         if constexpr (NEAR) __syncthreads();  // the window is no longer read
 
         if constexpr (LONG)
-            long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, (double *)smem, TRW * K,
+            long_phase<K, XT, P>(A, X, ldx, u0, min(TRW, A.M - u0), lq, pv, acc, (acc_t *)smem, TRW * K,
                                  MASK ? kv : K);
 
 #pragma unroll
@@ -1145,10 +1143,10 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
     }
 }
 
-// LDS: s_val[capv] | s_lsum[256*P] doubles | s_start[TRW+1]
+// LDS: s_val[capv] | s_lsum[256*P] | s_start[TRW+1]
 __host__ __device__ constexpr size_t lds_bytes(int capv, int P, int trw)
 {
-    return (size_t)lsum_at(capv) * 4 + (size_t)kWG * P * 8 + ((size_t)trw + 1) * 4;
+    return (size_t)lsum_at(capv) * 4 + (size_t)kWG * P * sizeof(acc_t) + ((size_t)trw + 1) * 4;
 }
 
 }  // namespace tcd

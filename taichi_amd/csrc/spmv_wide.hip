@@ -26,7 +26,7 @@ int launch_wide_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
     const int ntiles = (A.M + TRW - 1) / TRW;
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
-    // the x window (floats), reused for the long-row sums (doubles)
+    // the x window, reused for the long-row sums
     const bool near = K <= 8 && A.near;
     const size_t lds = (A.S > 0 || A.L > 0 || near) ? wide_lds_bytes(K, TRW, near, A.L > 0) : 0;
     const XT *x = (const XT *)X;
