@@ -251,7 +251,10 @@ typedef struct tc_request {
 /* batched SpMV over n requests on any mix of matrices: the requests are cut
    into column chunks of 8, 4, 2 and 1 columns and each (chunk width, x type)
    group is one kernel launch over all its tiles; each y is bit-identical to
-   what tc_spmv computes for that request alone */
+   what tc_spmv computes for that request alone. Asynchronous on `stream`:
+   the descriptors go through buffers the library owns (8 slots a device,
+   pinned host and device memory, each reused once its batch has run); the
+   call returns without waiting for the stream. */
 int tc_spmv_batch(const tc_request *reqs, int n, void *stream);
 
 /* ---- serving (no reference counterpart): resident matrices, admission by

@@ -575,15 +575,28 @@ class Matrix:
         return (c.rows_per_tile, c.lds_values, c.grid, c.flags), us.value
 
 
+class Batch:
+    """A fixed request set [(Matrix, X, Y), ...] (k <= 64 each) for repeated
+    batched launches: the C request array is built once, and each launch
+    is one tc_spmv_batch call (asynchronous on its stream). Holds the items,
+    so their tensors stay alive as long as the batch."""
+
+    def __init__(self, items):
+        self.items = list(items)
+        self.reqs = (Request * len(self.items))()
+        for i, (A, X, Y) in enumerate(self.items):
+            k = 1 if X.dim() == 1 else X.shape[1]
+            self.reqs[i] = Request(A.handle, X.data_ptr(), 1 if X.dim() == 1 else X.stride(0),
+                                   Matrix._dtype_of(X), Y.data_ptr(), 1 if Y.dim() == 1 else Y.stride(0), k)
+
+    def launch(self, stream=None):
+        check(lib().tc_spmv_batch(self.reqs, len(self.items), _torch_stream(stream)), "spmv_batch")
+
+
 def spmv_batch(items, stream=None):
     """Batched SpMV over [(Matrix, X, Y), ...] (k <= 64 each): one launch per
     (column chunk width, x type) group; each Y equals Matrix.spmv alone."""
-    reqs = (Request * len(items))()
-    for i, (A, X, Y) in enumerate(items):
-        k = 1 if X.dim() == 1 else X.shape[1]
-        reqs[i] = Request(A.handle, X.data_ptr(), 1 if X.dim() == 1 else X.stride(0),
-                          Matrix._dtype_of(X), Y.data_ptr(), 1 if Y.dim() == 1 else Y.stride(0), k)
-    check(lib().tc_spmv_batch(reqs, len(items), _torch_stream(stream)), "spmv_batch")
+    Batch(items).launch(stream)
 
 
 class Graph:

@@ -14,7 +14,9 @@ namespace tc {
 size_t batch_desc_bytes(int max_requests);
 // Fills the descriptors of n requests in host_desc (which must stay valid
 // until the copy on stream s has run) and enqueues their copy to dev_desc
-// plus one launch per (column chunk width, x type) group of the batch.
-int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s);
+// plus one launch per (column chunk width, x type) group of the batch. Both
+// buffers hold cap bytes; a batch whose descriptors need more is rejected
+// (TC_ERR_CAPACITY) before anything is written or enqueued.
+int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s, size_t cap);
 
 }  // namespace tc

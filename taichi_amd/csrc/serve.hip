@@ -60,6 +60,7 @@ struct Slot {
     hipEvent_t ev = nullptr;
     void *host_desc = nullptr;
     void *dev_desc = nullptr;
+    size_t desc_cap = 0;  // bytes of each descriptor buffer
     std::vector<ReqRec *> reqs;
     bool busy = false;
 };
@@ -137,7 +138,7 @@ void dispatcher(tc_server *S)
             if (alone) rc = tc_spmv(q.A, q.X, q.ldx, q.dtype, q.Y, q.ldy, q.k, sl.stream);
             else reqs[nsmall++] = q;
         }
-        if (rc == TC_OK && nsmall > 0) rc = tc::batch_enqueue(reqs.data(), nsmall, sl.host_desc, sl.dev_desc, sl.stream);
+        if (rc == TC_OK && nsmall > 0) rc = tc::batch_enqueue(reqs.data(), nsmall, sl.host_desc, sl.dev_desc, sl.stream, sl.desc_cap);
         if (rc == TC_OK && hipEventRecord(sl.ev, sl.stream) != hipSuccess) rc = TC_ERR_HIP;
         lk.lock();
         if (rc != TC_OK)
@@ -278,6 +279,7 @@ try {
     S->slots.resize(cfg->streams);
     const size_t db = tc::batch_desc_bytes(cfg->max_batch);
     for (Slot &sl : S->slots) {
+        sl.desc_cap = db;
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
             hipHostMalloc(&sl.host_desc, db, hipHostMallocDefault) != hipSuccess ||

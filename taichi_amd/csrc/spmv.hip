@@ -117,6 +117,36 @@ int check_spmv_args(const tc_matrix *A, const void *X, int ldx, int dtype, const
     return TC_OK;
 }
 
+// Descriptor buffers of tc_spmv_batch: kDescSlots slots per device, each a
+// pinned host buffer (the copy's source) and a device buffer (what the
+// kernels read), used in turn. A slot is refilled once the event recorded
+// after its batch's kernels has passed, so a call never waits for its
+// stream; only a caller with more than kDescSlots batches in flight waits,
+// for the oldest. (A stream-ordered allocation per call instead cost about
+// 0.3 ms a call: the default pool maps its memory again after each release.)
+constexpr int kDescSlots = 8;
+struct DescSlot {
+    void *host = nullptr;
+    void *dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+struct DescRing {
+    std::mutex mu;
+    DescSlot slot[kDescSlots];
+    int next = 0;
+};
+DescRing &desc_ring(int dev)
+{
+    static std::mutex mu;
+    static std::map<int, DescRing *> rings;  // one per device, kept until the process ends
+    std::lock_guard<std::mutex> g(mu);
+    DescRing *&r = rings[dev];
+    if (!r) r = new DescRing();
+    return *r;
+}
+
 }  // namespace tcd
 
 using namespace tcd;
@@ -395,13 +425,31 @@ try {
     if (n < 0 || (n > 0 && !reqs)) return tc::fail(TC_ERR_INVALID, "tc_spmv_batch: bad argument");
     if (n == 0) return TC_OK;
     hipStream_t s = (hipStream_t)stream;
-    std::vector<char> host(tc::batch_desc_bytes(n));
-    void *dd = nullptr;
-    HIPCHECK(hipMallocAsync(&dd, host.size(), s));
-    int rc = tc::batch_enqueue(reqs, n, host.data(), dd, s);
-    HIPCHECK(hipFreeAsync(dd, s));
-    // the host descriptor array must outlive the async copy
-    HIPCHECK(hipStreamSynchronize(s));
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    DescRing &R = desc_ring(dev);
+    std::lock_guard<std::mutex> g(R.mu);
+    DescSlot &sl = R.slot[R.next];
+    R.next = (R.next + 1) % kDescSlots;
+    if (sl.pending) {  // the copy that last read this slot
+        HIPCHECK(hipEventSynchronize(sl.ev));
+        sl.pending = false;
+    }
+    const size_t need = tc::batch_desc_bytes(n);
+    if (sl.cap < need) {
+        if (sl.host) (void)hipHostFree(sl.host);
+        if (sl.dev) (void)hipFree(sl.dev);
+        sl.host = sl.dev = nullptr;
+        sl.cap = 0;
+        if (hipHostMalloc(&sl.host, need, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&sl.dev, need) != hipSuccess)
+            return tc::fail(TC_ERR_NOMEM, "tc_spmv_batch: %zu B of descriptor memory", need);
+        sl.cap = need;
+    }
+    if (!sl.ev) HIPCHECK(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
+    const int rc = tc::batch_enqueue(reqs, n, sl.host, sl.dev, s, need);
+    HIPCHECK(hipEventRecord(sl.ev, s));
+    sl.pending = true;
     return rc;
 } catch (...) {
     return tc::exception_status(__func__);

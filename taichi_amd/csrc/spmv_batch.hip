@@ -206,7 +206,7 @@ int chunk_class(const tc_request &r, int ki, int K, int c0, int dt, int kv)
 // batch_desc_bytes(n) bytes, valid until the copy on s has run), copies them
 // to dev_desc and enqueues one launch per batch class present. Each
 // request's y equals what tc_spmv computes for it alone.
-int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s)
+int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc, hipStream_t s, size_t cap)
 {
     static const int kChunkK[4] = {1, 2, 4, 8};
     // chunks of a request: 8s; then the rest (1..7 columns) as one masked
@@ -252,6 +252,12 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
         });
     }
     if (nchunks == 0) return TC_OK;
+    {
+        const size_t need = (((size_t)nchunks * sizeof(WorkDesc) + 15) & ~(size_t)15) + (size_t)nchunks * sizeof(int);
+        if (need > cap)
+            return tc::fail(TC_ERR_CAPACITY, "batch: %d column chunks need %zu descriptor bytes, %zu available",
+                            nchunks, need, cap);
+    }
     int cls_begin[kBatchClasses + 1] = {0};
     for (int c = 0; c < kBatchClasses; c++) cls_begin[c + 1] = cls_begin[c] + cls_count[c];
     WorkDesc *D = (WorkDesc *)host_desc;
