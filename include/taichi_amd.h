@@ -256,6 +256,18 @@ typedef struct tc_request {
    pinned host and device memory, each reused once its batch has run); the
    call returns without waiting for the stream. */
 int tc_spmv_batch(const tc_request *reqs, int n, void *stream);
+/* tc_spmv_batch without any allocation. No counterpart in the reference: it
+   is for callers that cannot allocate (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc). The descriptor table lives
+   in a fixed 1 MiB array of device memory that each GPU the library runs on
+   holds until the process ends, staged through a static host buffer; a batch
+   whose descriptors need more (about 5,000 column chunks of at most 8
+   columns) is refused with TC_ERR_CAPACITY before anything is enqueued. All
+   calls on one GPU share that array and it keeps the last call's
+   descriptors: calls that overlap on one GPU (on different streams, before
+   the previous batch has run) race; calls on one stream are ordered. Same
+   results as tc_spmv_batch. */
+int tc_spmv_batch_static(const tc_request *reqs, int n, void *stream);
 
 /* ---- serving (no reference counterpart): resident matrices, admission by
    HBM footprint, continuous batching into persistent launches ---- */

@@ -155,6 +155,7 @@ def lib():
         "tc_autotune": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, vp,
                                   P(Config), P(C.c_float)]),
         "tc_spmv_batch": (C.c_int, [P(Request), C.c_int, vp]),
+        "tc_spmv_batch_static": (C.c_int, [P(Request), C.c_int, vp]),
         "tc_graph_capture": (C.c_int, [vp, vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int,
                                        vp, P(vp)]),
         "tc_graph_launch": (C.c_int, [vp, vp]),
@@ -591,6 +592,12 @@ class Batch:
 
     def launch(self, stream=None):
         check(lib().tc_spmv_batch(self.reqs, len(self.items), _torch_stream(stream)), "spmv_batch")
+
+    def launch_static(self, stream=None):
+        """The same launch through tc_spmv_batch_static (no allocation; the
+        descriptor table is the library's fixed per-GPU array)."""
+        check(lib().tc_spmv_batch_static(self.reqs, len(self.items), _torch_stream(stream)),
+              "spmv_batch_static")
 
 
 def spmv_batch(items, stream=None):

@@ -307,3 +307,48 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
 
 }  // namespace tc
 
+
+// ---- tc_spmv_batch_static: the batched launch without any allocation ----
+// The descriptor table lives in `scratch`, a fixed array of device memory
+// that each GPU holds once for the whole process, and the descriptors are
+// filled in a static host buffer; nothing is allocated on any call. In host
+// code `scratch` names a host-side stand-in, not device memory: every host
+// use goes through the address hipGetSymbolAddress gives for the current
+// device (each GPU has its own copy at its own address), taken once per
+// device at that device's first call.
+constexpr size_t kStaticDescBytes = 1u << 20;  // descriptor bytes a call may use, per GPU
+
+static __device__ __align__(256) char scratch[kStaticDescBytes];
+
+namespace {
+
+constexpr int kStaticMaxDevices = 64;
+
+std::mutex g_static_mu;                               // guards the staging buffer and the addresses
+void *g_static_dev[kStaticMaxDevices] = {};           // scratch's address on each device
+alignas(64) char g_static_host[kStaticDescBytes];     // host staging of the descriptors
+
+}  // namespace
+
+extern "C" int tc_spmv_batch_static(const tc_request *reqs, int n, void *stream)
+try {
+    if (n < 0 || (n > 0 && !reqs)) return tc::fail(TC_ERR_INVALID, "tc_spmv_batch_static: bad argument");
+    if (n == 0) return TC_OK;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kStaticMaxDevices)
+        return tc::fail(TC_ERR_UNSUPPORTED, "tc_spmv_batch_static: device %d beyond %d", dev, kStaticMaxDevices);
+    std::lock_guard<std::mutex> g(g_static_mu);
+    if (!g_static_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return tc::fail(TC_ERR_HIP, "tc_spmv_batch_static: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        g_static_dev[dev] = p;
+    }
+    // the staging buffer is pageable: the copy has read it when
+    // hipMemcpyAsync returns, so the next call may refill it
+    return tc::batch_enqueue(reqs, n, g_static_host, g_static_dev[dev], (hipStream_t)stream, kStaticDescBytes);
+} catch (...) {
+    return tc::exception_status(__func__);
+}
