@@ -1,4 +1,7 @@
 """Shared test helpers: workloads from the generators, device runs, checks."""
+import json
+import os
+
 import numpy as np
 
 import oracle as O
@@ -95,3 +98,45 @@ def morph_images():
         a[rng.random((512, W)) < 0.02] = 0
         out.append((f"wide{W}", a))
     return out
+
+
+# ---- hand-derived band goldens (tests/golden/make_golden.py, BANDS_NOTE) ----
+BAND_CASES = ["band_plus_2048", "band_minus_2048", "band_pair_2048"]
+
+
+def band_goldens():
+    """the hand-derived band partitions of tests/golden/band_partitions.json"""
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "band_partitions.json")) as f:
+        return json.load(f)["cases"]
+
+
+def band_csr(n, bands):
+    """the fixture's input: the union of bands (offset, first row, end row,
+    rows r % 7 == 3 dropped or not), columns ascending within a row"""
+    rows = [[] for _ in range(n)]
+    for o, r0, r1, gaps in bands:
+        for r in range(r0, r1):
+            if not (gaps and r % 7 == 3):
+                rows[r].append(r + o)
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([c for r in rows for c in sorted(r)], np.int32)
+    return rp, col
+
+
+def band_format_equals(fmt, want, rp, col):
+    """fmt equals a band golden's format; the aliased case keeps the input
+    CSR unencoded, the others hold the listed remainder entries"""
+    for k in O.Format.FIELDS:
+        assert getattr(fmt, k) == want[k], k
+    for k in ("xStart", "yStart", "xStop", "nDiasPtr", "neg_offsets"):
+        assert np.array_equal(np.asarray(getattr(fmt, k), np.int64).ravel(), np.array(want[k], np.int64)), k
+    if want["aliased"]:
+        assert np.array_equal(fmt.csrRow, rp) and np.array_equal(fmt.csrCol, col)
+        return
+    M = len(rp) - 1
+    rem = np.array(want["remainder"], np.int64).reshape(-1, 2)
+    crow = np.zeros(M + 1, np.int64)
+    crow[1:] = np.cumsum(np.bincount(rem[:, 0], minlength=M))
+    assert np.array_equal(fmt.csrRow, crow)
+    assert np.array_equal(fmt.csrCol, rem[:, 1])

@@ -10,7 +10,7 @@ import oracle as O
 import taichi_amd as T
 from taichi_amd import workloads as W
 
-from helpers import rmat
+from helpers import BAND_CASES, band_csr, band_format_equals, band_goldens, rmat
 
 pytestmark = pytest.mark.gpu
 
@@ -145,3 +145,73 @@ def test_duplicates_off_the_dense_diagonals_stay_on_the_device(dev):
     assert len(col2) > len(col)
     fmt = _device(n, n, rp2, col2, dev)
     _same_as_oracle(fmt, n, n, rp2, col2)
+
+
+# ---- the hand-derived band goldens (tests/golden/make_golden.py, BANDS_NOTE) ----
+
+
+@pytest.mark.parametrize("name", BAND_CASES)
+def test_device_build_equals_band_golden(name, dev):
+    """The contour-corner transposition pinned on the device build: one band
+    at +-2048 gives no shape; the pair gives two shapes, each from the other
+    band's contour."""
+    g = band_goldens()[name]
+    M, N = g["M"], g["N"]
+    rp, col = band_csr(M, g["bands"])
+    band_format_equals(_device(M, N, rp, col, dev), g["format"], rp, col)
+    ps, _ = T.partition_shapes(M, N, rp, col)
+    band_format_equals(ps, g["format"], rp, col)
+
+
+def _spmv_parity(M, N, rp, col, fmt, dev):
+    """integer x bit-exact against the CPU FP32 CSR sums at k = 1..8 (every
+    column) and through the batched launch; U[0,1) x within the FP64 bound"""
+    A = T.Matrix(fmt, nnz=len(col))
+    rng = np.random.default_rng(8)
+    for k in range(1, 9):
+        X = rng.integers(0, 8, (N, k)).astype(np.float32)
+        Xd = torch.from_numpy(X).to(dev)
+        Y = torch.full((M, k), float("nan"), device=dev)
+        if k == 1:
+            A.spmv(Xd[:, 0].contiguous(), Y[:, 0])
+        else:
+            A.spmv(Xd, Y)
+        torch.cuda.synchronize()
+        Yh = Y.cpu().numpy()
+        for c in range(k):
+            assert np.array_equal(Yh[:, c], O.csr_spmv_f32(rp, col, np.ascontiguousarray(X[:, c]))), (k, c)
+        Yb = torch.full((M, k), float("nan"), device=dev)
+        T.spmv_batch([(A, Xd if k > 1 else Xd[:, 0].contiguous(), Yb if k > 1 else Yb[:, 0])])
+        torch.cuda.synchronize()
+        assert torch.equal(Yb, Y), k
+    x = rng.random(N).astype(np.float32)
+    y = torch.empty(M, device=dev)
+    A.spmv(torch.from_numpy(x).to(dev), y)
+    torch.cuda.synchronize()
+    y64 = O.csr_spmv_f64(rp, col, x.astype(np.float64))
+    assert O.rel_err(y.cpu().numpy(), y64, O.csr_absrow_f64(rp, col, x.astype(np.float64))) <= 1e-6
+    return A.stats()
+
+
+def test_spmv_on_band_pair_golden(dev):
+    """Two shapes of one dense offset each (the reference's hybrid2 class,
+    TaiChi.cu:1044-1129 dispatched at :1288-1290), rows covered by both
+    shapes, zero cells on one of them"""
+    g = band_goldens()["band_pair_2048"]
+    M, N = g["M"], g["N"]
+    rp, col = band_csr(M, g["bands"])
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    assert (fmt.dia_shapes, fmt.num_negs) == (2, 2)
+    st = _spmv_parity(M, N, rp, col, fmt, dev)
+    # shape 0's zero cells lie in rows shape 1 also covers: the layout turns
+    # them into bitmap bits (an FP32 cancellation there would be inexact)
+    assert st["bitmap_shapes"] >= 1 and st["holes"] == 875
+
+
+def test_spmv_on_three_bands(dev):
+    """three shapes sharing rows (S = 3), each with several dense offsets"""
+    rp, col = _three_bands()
+    n = 60000
+    fmt, _ = T.partition_shapes(n, n, rp, col)
+    assert fmt.dia_shapes == 3
+    _spmv_parity(n, n, rp, col, fmt, dev)

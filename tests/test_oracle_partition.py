@@ -12,6 +12,8 @@ import pytest
 import oracle as O
 import taichi_amd as T
 
+from helpers import BAND_CASES, band_csr, band_format_equals, band_goldens
+
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -154,6 +156,48 @@ def test_golden_diag_gaps_product_partition():
     rp, col = np.array(g["rowptr"], np.int32), np.array(g["col"], np.int32)
     fmt, _ = T.partition_from_image(g["M"], g["N"], rp, col, img=O.thumbnail(g["M"], g["N"], rp, col))
     _format_equals(fmt, g["format"])
+
+
+# ---- band goldens pinning the contour-corner transposition ----
+# (hand derivations in tests/golden/make_golden.py, BANDS_NOTE)
+
+
+@pytest.mark.parametrize("name", BAND_CASES)
+def test_golden_bands_oracle_pipeline(name):
+    g = band_goldens()[name]
+    M, N = g["M"], g["N"]
+    rp, col = band_csr(M, g["bands"])
+    img = O.thumbnail(M, N, rp, col)
+    cs = O.find_contours(255 - img)
+    assert len(cs) == len(g["contours"])
+    for (pts, _), want, sh in zip(cs, g["contours"], g["shapes"]):
+        assert len(pts) == want["len"]
+        assert list(pts[0]) == want["first"] and list(pts[len(pts) // 2]) == want["mid"]
+        # the corners' x (a column) is the square's row start: x1 = P0.x
+        x1, y1 = want["first"]
+        a1, b1 = want["mid"][0] - x1, want["mid"][1] - y1
+        assert O.cal_shape_size(M, N, 512, x1, y1, a1, b1) == (sh["x"], sh["y"], sh["a"], sh["b"], sh["gain"])
+        assert O.width_for(min(sh["a"], sh["b"])) == sh["width"]
+    fmt, _ = O.partition(M, N, rp, col)
+    band_format_equals(fmt, g["format"], rp, col)
+
+
+@pytest.mark.parametrize("name", BAND_CASES)
+def test_golden_bands_product_partition(name):
+    g = band_goldens()[name]
+    rp, col = band_csr(g["M"], g["bands"])
+    fmt, _ = T.partition_from_image(g["M"], g["N"], rp, col, img=O.thumbnail(g["M"], g["N"], rp, col))
+    band_format_equals(fmt, g["format"], rp, col)
+
+
+def test_golden_band_pair_interpreter_equals_csr():
+    """On the two-shape golden the format's meaning (each shape's dense
+    offset plus the signed remainder) gives the CSR sums exactly."""
+    g = band_goldens()["band_pair_2048"]
+    rp, col = band_csr(g["M"], g["bands"])
+    fmt, _ = O.partition(g["M"], g["N"], rp, col)
+    x = np.random.default_rng(4).integers(0, 8, g["N"]).astype(np.float32)
+    assert np.array_equal(O.taichi_spmv(fmt, x, np.float32), O.csr_spmv_f32(rp, col, x))
 
 
 # ---- the 0.6 density rule (TaiChi.cu:756-757) on both sides of it ----
