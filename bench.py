@@ -254,8 +254,10 @@ def _wall(fn, reps=3):
 def bench_partition(n, rp, col, dev):
     """Partition + format build of the headline matrix: the device build
     (tc_partition_device on resident rowptr/col, format copied back), the
-    whole partition_shapes call (CSR upload included), and the host build
-    given the same thumbnail. Device and host formats must be identical."""
+    whole partition_shapes call (CSR upload included), the same call with
+    csrVal (FLT_MAX markers written back, as main.cu calls it), and the host
+    build given the same thumbnail. Device and host formats must be
+    identical."""
     import ctypes as C
     import torch
     import taichi_amd as T
@@ -264,6 +266,9 @@ def bench_partition(n, rp, col, dev):
     dev_fmt = T.partition_device(n, n, rp, col, rpd, cold)
     t_dev = _wall(lambda: T.partition_device(n, n, rp, col, rpd, cold))
     t_shapes = _wall(lambda: T.partition_shapes(n, n, rp, col))
+    # as main.cu:45 calls it: with csrVal, whose claimed entries get FLT_MAX
+    ones = np.ones(len(col), np.float32)
+    t_val = _wall(lambda: T.partition_shapes(n, n, rp, col, val=ones))
     H, W_, pm, pn = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     T.lib().tc_thumb_dims(n, n, C.byref(H), C.byref(W_), C.byref(pm), C.byref(pn))
     img = torch.empty(H.value * W_.value, dtype=torch.uint8, device=dev)
@@ -274,6 +279,7 @@ def bench_partition(n, rp, col, dev):
     same = all(getattr(dev_fmt, f) == getattr(host_fmt, f) for f in T.Format.FIELDS) and \
         all(np.array_equal(getattr(dev_fmt, f), getattr(host_fmt, f)) for f in T.Format.ARRAYS)
     return {"device_build_s": round(t_dev, 4), "partition_shapes_s": round(t_shapes, 4),
+            "partition_shapes_with_val_s": round(t_val, 4),
             "host_build_s": round(t_host, 4), "speedup": round(t_host / t_dev, 2),
             "device_equals_host": bool(same), "remainder_entries": int(dev_fmt.csrRow[-1])}
 

@@ -39,4 +39,10 @@ void candidate_squares_from_pts(int M, int N, const std::vector<int> &pts, std::
 void decide_dense(int side, int w, const long long *cnt, DiaShape &sh, std::vector<int> &dense_of);
 void shape_arrays(const std::vector<DiaShape> &kept, int nnz, TaiChi_new *out);
 
+// tc_partition_device, and when claimed (nnz bytes, host) is not null it
+// receives 1 for every entry a dense diagonal claimed: the entries the
+// reference marks csrVal[j] = FLT_MAX (TaiChi.cu:866). partition_gpu.hip.
+int partition_device_claims(int M, int N, int nnz, int *rowptr, int *col, const int *d_rowptr, const int *d_col,
+                            TaiChi_new *out, void *stream, uint8_t *claimed);
+
 }  // namespace tc

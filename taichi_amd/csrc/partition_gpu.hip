@@ -171,9 +171,10 @@ __global__ void narrow_rows(int n, const long long *__restrict__ a, int *__restr
     } while (0)
 
 // The passes above for the squares of one thumbnail; fills out like
-// tc_partition_from_image with a NULL val.
+// tc_partition_from_image with a NULL val, and claimed_out (nnz bytes, host;
+// may be null) with each entry's claim byte.
 int partition_on_device(int M, int N, int nnz, int *rowptr, int *col, const int *d_rowptr, const int *d_col,
-                        const std::vector<int> &pts, TaiChi_new *out, hipStream_t s)
+                        const std::vector<int> &pts, TaiChi_new *out, hipStream_t s, uint8_t *claimed_out)
 {
     std::vector<tc::Square> cand;
     tc::candidate_squares_from_pts(M, N, pts, cand);
@@ -242,6 +243,10 @@ int partition_on_device(int M, int N, int nnz, int *rowptr, int *col, const int 
     DCHECK(hipStreamSynchronize(s));
     if (hdup) return tc::fail(TC_ERR_UNSUPPORTED, "partition (device): duplicate entries on a dense diagonal");
 
+    if (claimed_out && nnz > 0) {
+        DCHECK(hipMemcpyAsync(claimed_out, claimed.p, (size_t)nnz, hipMemcpyDeviceToHost, s));
+        DCHECK(hipStreamSynchronize(s));
+    }
     memset(out, 0, sizeof *out);
     tc::shape_arrays(kept, nnz, out);
     if (out->dense_nnz == 0) {  // no dense diagonal: the input CSR is the format (TaiChi.cu:575-578)
@@ -322,9 +327,9 @@ int partition_on_device(int M, int N, int nnz, int *rowptr, int *col, const int 
 
 }  // namespace
 
-extern "C" int tc_partition_device(int M, int N, int nnz, int *rowptr, int *col, const int *d_rowptr,
-                                   const int *d_col, TaiChi_new *out, void *stream)
-try {
+int tc::partition_device_claims(int M, int N, int nnz, int *rowptr, int *col, const int *d_rowptr, const int *d_col,
+                                TaiChi_new *out, void *stream, uint8_t *claimed)
+{
     if (M < 0 || N < 0 || nnz < 0 || !rowptr || (nnz > 0 && !col) || !out || (M > 0 && !d_rowptr) ||
         (nnz > 0 && !d_col))
         return tc::fail(TC_ERR_INVALID, "tc_partition_device: bad argument");
@@ -341,7 +346,13 @@ try {
         if (rc == TC_OK) rc = tc::sdf_on_device(dimg.as<unsigned char>(), H, W, pts, s);
         if (rc != TC_OK) return rc;
     }
-    return partition_on_device(M, N, nnz, rowptr, col, d_rowptr, d_col, pts, out, s);
+    return partition_on_device(M, N, nnz, rowptr, col, d_rowptr, d_col, pts, out, s, claimed);
+}
+
+extern "C" int tc_partition_device(int M, int N, int nnz, int *rowptr, int *col, const int *d_rowptr,
+                                   const int *d_col, TaiChi_new *out, void *stream)
+try {
+    return tc::partition_device_claims(M, N, nnz, rowptr, col, d_rowptr, d_col, out, stream, nullptr);
 } catch (...) {
     return tc::exception_status(__func__);
 }

@@ -10,9 +10,12 @@
 // OR is order independent, so the image is bit-exact against the closed form.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+#include <cmath>
 #include <vector>
 
 #include "common.h"
+#include "partition.h"
 
 namespace {
 
@@ -152,9 +155,24 @@ try {
         HIPCHECK(hipMalloc((void **)&b.dimg, img.size()));
         HIPCHECK(hipMemcpy(b.drow, rowptr, ((size_t)M + 1) * sizeof(int), hipMemcpyHostToDevice));
         if (nnz) HIPCHECK(hipMemcpy(b.dcol, col, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
+        // the device build; with val, its claim bytes become the FLT_MAX
+        // markers the reference writes (TaiChi.cu:866). An input whose val
+        // already holds markers (entries the reference counts as claimed
+        // from the start, :730) takes the host build, as do duplicates on a
+        // dense diagonal (the device build reports them as unsupported).
+        bool premarked = false;
+        if (val)
+            for (int j = 0; j < nnz && !premarked; j++) premarked = !(std::fabs(val[j] - FLT_MAX) > 1e-8);
         int rc = TC_ERR_UNSUPPORTED;
-        if (!val) rc = tc_partition_device(M, N, nnz, rowptr, col, b.drow, b.dcol, out, nullptr);
-        const bool host_build = rc == TC_ERR_UNSUPPORTED;  // val given, or duplicates on a dense diagonal
+        if (!premarked) {
+            std::vector<uint8_t> cl(val ? (size_t)nnz : 0);
+            rc = tc::partition_device_claims(M, N, nnz, rowptr, col, b.drow, b.dcol, out, nullptr,
+                                             val ? cl.data() : nullptr);
+            if (rc == TC_OK && val)
+                for (int j = 0; j < nnz; j++)
+                    if (cl[j]) val[j] = FLT_MAX;
+        }
+        const bool host_build = rc == TC_ERR_UNSUPPORTED;
         if (host_build) {
             rc = tc_thumbnail(M, N, b.drow, b.dcol, b.dimg, nullptr);
             if (rc == TC_OK) HIPCHECK(hipMemcpy(img.data(), b.dimg, img.size(), hipMemcpyDeviceToHost));
