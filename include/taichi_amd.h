@@ -278,10 +278,10 @@ typedef struct tc_server_config {
     int streams;              /* launch slots that may run concurrently */
     int max_inflight;         /* requests on the GPU at once */
     int launch;               /* how a batch is launched on its slot's stream:
-                                 0 (auto): requests of at least 32768 rows one
-                                 by one (tc_spmv), the others as one batched
-                                 launch group (tc_spmv_batch); 1: all batched;
-                                 2: all one by one. Same results either way. */
+                                 0 and 1: as one batched launch group (the
+                                 requests coalesced, tc_spmv_batch's kernels);
+                                 2: one by one (tc_spmv). Same results either
+                                 way. */
 } tc_server_config;
 typedef struct tc_server_stats {
     long long resident_bytes;

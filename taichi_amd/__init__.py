@@ -657,9 +657,8 @@ class Server:
     """Continuous-batching SpMV server over resident matrices (tc_server)."""
 
     def __init__(self, hbm_cap_bytes=200 << 30, max_batch=64, streams=3, max_inflight=256, launch=0):
-        """launch 0 (auto): requests of at least 32768 rows run as their own
-        launches, the rest as one batched launch group; 1: all batched; 2: all
-        one by one. Same results either way."""
+        """launch 0 or 1: each batch of requests runs as one coalesced
+        batched launch group; 2: one by one. Same results either way."""
         h = C.c_void_p()
         cfg = ServerConfig(hbm_cap_bytes, max_batch, streams, max_inflight, launch)
         check(lib().tc_server_create(C.byref(cfg), C.byref(h)), "server_create")
