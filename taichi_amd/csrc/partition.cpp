@@ -195,6 +195,21 @@ void tc::partition_square(const int *rowptr, const int *col, std::vector<uint8_t
 
 extern "C" {
 
+// thumbnail sizing (morohutils.cu:191-200); the thumbnail kernel is in thumb.hip
+int tc_thumb_dims(int M, int N, int *H, int *W, int *pM, int *pN)
+{
+    if (!H || !W || !pM || !pN) return tc::fail(TC_ERR_INVALID, "tc_thumb_dims: null argument");
+    *H = kThumbH;
+    *W = *pM = *pN = 0;
+    if (M <= kThumbH) return 1;  // morohutils.cu:192-196
+    *pM = M / kThumbH;
+    const double scale = (double)kThumbH / (double)M;
+    *W = (int)(scale * N);       // double product, truncated (:198-199)
+    if (*W <= 0) return 2;       // the reference would divide by zero at :200
+    *pN = N / *W;
+    return 0;
+}
+
 void tc_cal_shape_size(int M, int N, int mF, int x1, int y1, int a1, int b1, int *x, int *y, int *a,
                        int *b, float *gain)
 {

@@ -83,20 +83,6 @@ __global__ __launch_bounds__(kWG) void thumb_expand(const unsigned *__restrict__
         if (e_ != hipSuccess) return tc::fail(TC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));          \
     } while (0)
 
-extern "C" int tc_thumb_dims(int M, int N, int *H, int *W, int *pM, int *pN)
-{
-    if (!H || !W || !pM || !pN) return tc::fail(TC_ERR_INVALID, "tc_thumb_dims: null argument");
-    *H = kThumbH;
-    *W = *pM = *pN = 0;
-    if (M <= kThumbH) return 1;  // morohutils.cu:192-196
-    *pM = M / kThumbH;
-    const double scale = (double)kThumbH / (double)M;
-    *W = (int)(scale * N);       // double product, truncated (:198-199)
-    if (*W <= 0) return 2;       // the reference would divide by zero at :200
-    *pN = N / *W;
-    return 0;
-}
-
 extern "C" int tc_thumbnail(int M, int N, const int *d_rowptr, const int *d_col, unsigned char *d_img, void *stream)
 try {
     int H, W, pM, pN;
