@@ -1016,26 +1016,28 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                                     for (int m = 0; m < CPL; m++) accum(acc[i * CPL + m], q[i][m]);
                             }
                         }
-                        __syncthreads();
-                        return;
-                    }
-                    for (int n = sh.ob; n < sh.oe; n++) {
-                        const int o = cload(A.offs, n);
+                    } else {
+                        // some row of this wavefront is off the square or
+                        // its columns: every term is tested
+                        for (int n = sh.ob; n < sh.oe; n++) {
+                            const int o = cload(A.offs, n);
 #pragma unroll
-                        for (int i = 0; i < RI; i++) {
-                            const int r = u0 + GPW * i + g;
-                            const int c = r + shift + o;
-                            if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
-                                const int j = n - sh.ob;
-                                const uint32_t w = j < 32 ? plo[i] : phi[i];
-                                float q[CPL];
-                                lds_get<CPL>(smem + kWideZero + (c - w0) * K + c0, q);
+                            for (int i = 0; i < RI; i++) {
+                                const int r = u0 + GPW * i + g;
+                                const int c = r + shift + o;
+                                if (r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side) {
+                                    const int j = n - sh.ob;
+                                    const uint32_t w = j < 32 ? plo[i] : phi[i];
+                                    float q[CPL];
+                                    lds_get<CPL>(smem + kWideZero + (c - w0) * K + c0, q);
 #pragma unroll
-                                for (int m = 0; m < CPL; m++)
-                                    accum(acc[i * CPL + m], PM ? keep_if(q[m], w, j & 31) : q[m]);
+                                    for (int m = 0; m < CPL; m++)
+                                        accum(acc[i * CPL + m], PM ? keep_if(q[m], w, j & 31) : q[m]);
+                                }
                             }
                         }
                     }
+                    // one barrier for both branches (the window is reused)
                     __syncthreads();
                     return;
                 }

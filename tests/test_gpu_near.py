@@ -98,3 +98,48 @@ def test_near_window_stays_off_for_scattered_and_light_remainders(dev):
                        (4096, *T.gen_banded(4096, 6, 0.5, 1))):
         fmt, _ = T.partition_shapes(M, M, rp, col)
         assert T.Matrix(fmt, nnz=len(col)).stats()["near_window"] == 0
+
+
+def _dense_and_near(n, seed):
+    """Dense diagonals -6..5 at density 0.75 (a shape whose zero cells become
+    presence-bitmap bits) inside a sparse band of offsets -60..59 at density
+    0.3 (about 32 remainder entries a row, all within 64 rows: the near
+    window)"""
+    rng = np.random.default_rng(seed)
+    r = np.repeat(np.arange(n, dtype=np.int64), 120)
+    o = np.tile(np.arange(-60, 60, dtype=np.int64), n)
+    c = r + o
+    dense = (o >= -6) & (o <= 5)
+    keep = (c >= 0) & (c < n) & (rng.random(len(r)) < np.where(dense, 0.75, 0.3))
+    r, c = r[keep], c[keep]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    return np.cumsum(rp).astype(np.int32), c.astype(np.int32)
+
+
+@pytest.mark.parametrize("grid", [0, 7])
+def test_near_window_with_bitmap_shape(grid, dev):
+    """A tile stages its shape's x window (after kWideZero floats of +0.0
+    that absent terms read) and then the near window over the same LDS; with
+    a grid of 7 workgroups each walks many tiles, so the next tile's shape
+    restages the window and its zero slot after the near window. Integer x
+    must give the CPU sums exactly at every k."""
+    n = 40000
+    rp, col = _dense_and_near(n, 17)
+    fmt, _ = T.partition_shapes(n, n, rp, col)
+    A = T.Matrix(fmt, nnz=len(col))
+    st = A.stats()
+    assert st["near_window"] == 1 and st["dia_shapes"] >= 1 and st["bitmap_shapes"] >= 1
+    if grid:
+        for k in range(2, 9):
+            A.set_config(k, T.F32, 0, 0, grid, 0)
+    xi = np.random.default_rng(3).integers(0, 8, (n, 8)).astype(np.float32)
+    X8 = torch.from_numpy(xi).to(dev)
+    want = [O.csr_spmv_f32(rp, col, np.ascontiguousarray(xi[:, c])) for c in range(8)]
+    for k in range(2, 9):
+        Y = torch.full((n, 8), float("nan"), device=dev)[:, :k]
+        A.spmv(X8[:, :k], Y)
+        torch.cuda.synchronize()
+        Yh = Y.cpu().numpy()
+        for c in range(k):
+            assert np.array_equal(Yh[:, c], want[c]), (k, c)

@@ -354,12 +354,17 @@ def test_taichi_spmv_host_path(dev):
     assert s_ms > 0
 
 
-@pytest.mark.parametrize("case", ["banded", "long"])
-def test_masked_column_chunks(case, dev):
+@pytest.mark.parametrize("case,dt", [("banded", torch.float32), ("banded", torch.bfloat16),
+                                     ("banded", torch.float16), ("long", torch.float32)],
+                         ids=["banded-f32", "banded-bf16", "banded-f16", "long-f32"])
+def test_masked_column_chunks(case, dt, dev):
     """k = 3, 5, 6, 7 (and 1..8) columns of an 8-column buffer run as one
     masked wide-rows launch: columns past k are neither read nor written,
     each column equals the single-vector result, and the batched launch over
-    the same views equals the plain one."""
+    the same views equals the plain one. With 2-byte x one lane owns all 8
+    columns, so 5, 6 and 7 columns are one masked 8-wide pass (its last-row
+    loads, and stores of the first 5 or 6 columns) also on a band with more
+    than two dense diagonals, in tc_spmv and in the batch."""
     if case == "banded":
         M = N = 20011
         rp, col = T.gen_banded(M, 13, 0.7, 61)
@@ -368,7 +373,10 @@ def test_masked_column_chunks(case, dev):
         rp, col = _local_mix(M, 1, 40, 509, 200, 20000, 62)
     fmt = host_format(M, N, rp, col)
     A = T.Matrix(fmt)
-    X8 = torch.from_numpy(T.gen_vector(N * 8, 1, 70).reshape(N, 8)).to(dev)
+    if case == "banded":
+        assert A.stats()["num_negs"] > 2
+    # integers 0..7: exact in BF16 and FP16 too
+    X8 = torch.from_numpy(T.gen_vector(N * 8, 1, 70).reshape(N, 8)).to(dev).to(dt)
     cols1 = []
     for c in range(8):
         y = torch.empty(M, device=dev)
@@ -383,7 +391,8 @@ def test_masked_column_chunks(case, dev):
         assert torch.isnan(Y8[:, k:]).all(), k                # nothing written past k
         for c in range(k):
             assert torch.equal(Y8[:, c], cols1[c]), (k, c)
-        assert np.array_equal(Y8[:, 0].cpu().numpy(), O.csr_spmv_f32(rp, col, X8[:, 0].cpu().numpy()))
+        assert np.array_equal(Y8[:, 0].cpu().numpy(),
+                              O.csr_spmv_f32(rp, col, X8[:, 0].float().cpu().numpy()))
         Yb = torch.full((M, 8), float("nan"), device=dev)
         items.append((A, X8[:, :k], Yb[:, :k], Y8, Yb, k))
     T.spmv_batch([(a, x, y) for a, x, y, _, _, _ in items])
@@ -403,7 +412,7 @@ def test_masked_column_chunks(case, dev):
     # masked path over a buffer that ends at column k of X's last row, with
     # NaN in every slot past column k: a masked chunk must not use them
     for k in (1, 3, 5, 6, 7):
-        flat = torch.full(((N - 1) * 8 + k,), float("nan"), device=dev)
+        flat = torch.full(((N - 1) * 8 + k,), float("nan"), device=dev, dtype=dt)
         Xs = flat.as_strided((N, k), (8, 1))
         Xs.copy_(X8[:, :k])
         Ys = torch.full((M, 8), float("nan"), device=dev)[:, :k]
