@@ -45,15 +45,20 @@ def cpu_baseline(rp, col, x, seconds=10.0):
     the 16,384-row five-diagonal matrix and the 2048^2 stencil at k = 1."""
     import oracle
     from taichi_amd import workloads as W
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    # the multi-core leg uses the host threads one GPU gets on the box (16),
+    # or fewer if the affinity mask allows fewer
+    affinity = len(os.sched_getaffinity(0))
+    threads = max(1, min(16, affinity))
     nnz = len(col)
     med, runs = _cpu_median(lambda: oracle.csr_spmv_f32(rp, col, x), seconds)
     med_mt, runs_mt = _cpu_median(lambda: oracle.csr_spmv_f32_mt(rp, col, x, threads), seconds / 4)
     out = {"value": nnz / med / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
            "sample": f"full R-MAT 2^22 matrix ({nnz} nnz), {runs} sequential FP32 "
                      f"CSR SpMVs, median {med * 1e3:.2f} ms",
-           "multi_core": {"value": nnz / med_mt / 1e9, "cores": threads,
-                          "sample": f"{runs_mt} SpMVs, rows split statically, median {med_mt * 1e3:.2f} ms"},
+           "multi_core": {"value": nnz / med_mt / 1e9, "cores": threads, "affinity_cores": affinity,
+                          "sample": f"{runs_mt} SpMVs, rows split statically over {threads} threads "
+                                    f"(the CPU share of one GPU; {affinity} in the affinity mask), "
+                                    f"median {med_mt * 1e3:.2f} ms"},
            "configs": {}}
     for name, (M, N, rp2, col2) in (("diag5_random_16384", W.diag5_random()),
                                     ("stencil5_2048", W.stencil5())):
@@ -67,7 +72,9 @@ def cpu_baseline(rp, col, x, seconds=10.0):
 
 
 def read_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/<name>)."""
+    """HBM bytes per launch of the headline kernel, from the committed
+    rocprofv3 --pmc summary profiles/<name> (FETCH_SIZE and WRITE_SIZE runs,
+    copy-calibrated); bench.py does not run the profiler itself."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -510,6 +517,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic("spmv_rmat22_traffic.txt"),
+                     "traffic_source": "profiles/spmv_rmat22_traffic.txt: rocprofv3 --pmc FETCH_SIZE, "
+                                       "then WRITE_SIZE, on this kernel and matrix, copy-calibrated; "
+                                       "read from the committed summary, not measured in this run",
                      "algorithmic_bytes": B},
         "timing": "hipGraph of the K SpMVs, HIP events on its stream",
         "eager_ms_per_step": round(ms_eager, 5),

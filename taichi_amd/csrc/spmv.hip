@@ -48,7 +48,10 @@ tc_config default_config(int K, int esize, long long remainder, long long diag_t
                                   : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
-    c.flags = K == 1 ? kQuadOff : 0;  // quad rows measured no faster on R-MAT; the autotuner tries both
+    // k = 1: one row per lane for FP32 x (quad rows measured no faster on
+    // R-MAT), quad rows for 2-byte x (2048^2 stencil, BF16: 34.7 against
+    // 40.1 us); the autotuner tries both
+    c.flags = K == 1 && esize == 4 ? kQuadOff : 0;
     return c;
 }
 
