@@ -780,21 +780,81 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
     }
 }
 
-// x row segment of a masked chunk: elements at or past nv are returned as 0.
-// The whole Q-wide vector is loaded (no divergence across the lane group):
-// it lies inside the row's ldx span (ldx is a multiple of Q and the lane's
-// first column is below kv <= ldx), except in the last row of X, which may
-// end at column kv -- there (last) only the first nv elements are read.
+// 16 bytes of zeros, the in-bounds stand-in for a vector load that would run
+// past the end of X (global, not constant, memory: a select between it and an
+// x address stays a global load)
+static __device__ __align__(16) uint32_t kXPad[4] = {0u, 0u, 0u, 0u};
+
+// A raw x row segment of Q elements as loaded: Q words for 4-byte x, Q/2
+// words (two elements each) for 2-byte x. Gathers fill segments and the adds
+// convert them (seg_at), so no conversion sits between one gather and the
+// next and a lane's gathers stay in flight together.
 template <typename XT, int Q>
-__device__ __forceinline__ void load_qm(const XT *__restrict__ p, int nv, bool last, float (&o)[Q])
+struct XSeg {
+    static constexpr int NW = sizeof(XT) == 4 ? Q : Q / 2;
+    uint32_t w[NW];
+};
+
+// NW consecutive words (global or LDS): one 4-, 8- or 16-byte access
+template <int NW>
+__device__ __forceinline__ void ld_words(const void *p, uint32_t (&w)[NW])
 {
-    if (!last || nv >= Q) {
-        load_q<XT, Q>(p, o);
-#pragma unroll
-        for (int m = 0; m < Q; m++) o[m] = m < nv ? o[m] : 0.0f;
+    static_assert(NW == 1 || NW == 2 || NW == 4, "x segments are 4, 8 or 16 bytes");
+    if constexpr (NW == 4) {
+        const uint4 v = *(const uint4 *)p;
+        w[0] = v.x, w[1] = v.y, w[2] = v.z, w[3] = v.w;
+    } else if constexpr (NW == 2) {
+        const uint2 v = *(const uint2 *)p;
+        w[0] = v.x, w[1] = v.y;
     } else {
+        w[0] = *(const uint32_t *)p;
+    }
+}
+template <int NW>
+__device__ __forceinline__ void st_words(void *p, const uint32_t (&w)[NW])
+{
+    static_assert(NW == 1 || NW == 2 || NW == 4, "x segments are 4, 8 or 16 bytes");
+    if constexpr (NW == 4) *(uint4 *)p = make_uint4(w[0], w[1], w[2], w[3]);
+    else if constexpr (NW == 2) *(uint2 *)p = make_uint2(w[0], w[1]);
+    else *(uint32_t *)p = w[0];
+}
+
+// element m (a constant after unrolling) of a segment, as a float
+template <typename XT, int Q>
+__device__ __forceinline__ float seg_at(const XSeg<XT, Q> &s, int m)
+{
+    if constexpr (sizeof(XT) == 4) return __uint_as_float(s.w[m]);
+    else if constexpr (__is_same(XT, bf16_t))
+        return __uint_as_float((m & 1) ? (s.w[m >> 1] & 0xffff0000u) : (s.w[m >> 1] << 16));
+    else return (float)__builtin_bit_cast(_Float16, (uint16_t)(s.w[m >> 1] >> (16 * (m & 1))));
+}
+
+template <typename XT, int Q>
+__device__ __forceinline__ void load_seg(const XT *__restrict__ p, XSeg<XT, Q> &s)
+{
+    ld_words(p, s.w);
+}
+
+// Segment of a masked chunk. The whole vector is loaded: it lies inside the
+// row's ldx span (ldx is a multiple of Q and the lane's first column is below
+// kv <= ldx), except in the last row of X, which may end at column kv --
+// there (last, nv < Q) the vector comes from kXPad and only the first nv
+// elements are read, one by one; the others are 0. Elsewhere the elements at
+// or past nv are whatever X holds there: they feed only columns that are
+// never stored. The vector load is unconditional (an address select); only
+// the rare tail is a branch.
+template <typename XT, int Q>
+__device__ __forceinline__ void load_seg_m(const XT *__restrict__ p, int nv, bool last, XSeg<XT, Q> &s)
+{
+    const bool tail = last && nv < Q;
+    ld_words(tail ? (const void *)kXPad : (const void *)p, s.w);
+    if (tail) {
 #pragma unroll
-        for (int m = 0; m < Q; m++) o[m] = m < nv ? load_x<XT>(p, m) : 0.0f;
+        for (int m = 0; m < Q; m++)
+            if (m < nv) {
+                if constexpr (sizeof(XT) == 4) s.w[m] = ((const uint32_t *)p)[m];
+                else s.w[m >> 1] |= (uint32_t)((const uint16_t *)p)[m] << (16 * (m & 1));
+            }
     }
 }
 
@@ -838,8 +898,8 @@ __host__ __device__ constexpr int wide_cpl(int K, int esize) { return esize == 2
 // butterfly) meet in LDS. Same canonical per-(row, column) order as every
 // other kernel. Requires X, Y, ldx and ldy aligned to CPL elements.
 // MASK: only the first kv < K columns exist; a lane computes all its
-// columns but keeps only those below kv (load_qm zeroes the rest; stores are
-// masked). A masked chunk runs k = 3, 5, 6, 7 as one pass of width 4 or 8
+// columns but stores only those below kv (load_seg_m keeps every load in
+// bounds). A masked chunk runs k = 3, 5, 6, 7 as one pass of width 4 or 8
 // instead of 2 + 1, 4 + 1, 4 + 2, 4 + 2 + 1 passes. Needs kv > K/2 (so a
 // lane's vector stays inside its X row, or X's last row is handled apart).
 // LDS: the x window [(TRW + kWideHalo) * K] floats, reused for the rows
@@ -859,9 +919,10 @@ __host__ __device__ constexpr size_t wide_lds_bytes(int K, int trw, bool near, b
     return win > lsum ? win : lsum;
 }
 
-// Stage x rows [w0, w0 + nrows), all K columns, in LDS as smem[(row - w0) * K
-// + col], CPL-wide vectors, four loads in flight per lane (masked chunks: the
-// columns at or past kv read as 0, as the lane's own loads would give them)
+// Stage x rows [w0, w0 + nrows), all K columns, in LDS as floats smem[(row -
+// w0) * K + col], CPL-wide vectors, four loads in flight per lane. Lanes past
+// the window repeat its last vector (same address, same value), so neither
+// the loads nor the LDS stores sit behind a branch.
 template <int K, typename XT, int CPL, bool MASK>
 __device__ __forceinline__ void stage_x_rows(const XT *__restrict__ X, int ldx, int N, int w0, int nrows, int kv,
                                              float *smem)
@@ -870,19 +931,52 @@ __device__ __forceinline__ void stage_x_rows(const XT *__restrict__ X, int ldx, 
     const int nw = nrows * LPR;  // CPL-wide vectors of the window
     const int tid = threadIdx.x;
     for (int b = 0; b < nw; b += kWG * 4) {
-        float tv[4][CPL];
+        XSeg<XT, CPL> tv[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int v = min(b + u * kWG + tid, nw - 1);
             const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
             if constexpr (MASK)
-                load_qm<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
-            else load_q<XT, CPL>(xp, tv[u]);
+                load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
+            else load_seg<XT, CPL>(xp, tv[u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int v = b + u * kWG + tid;
-            if (v < nw) lds_put<CPL>(smem + (v / LPR) * K + (v % LPR) * CPL, tv[u]);
+            const int v = min(b + u * kWG + tid, nw - 1);
+            float f[CPL];
+#pragma unroll
+            for (int m = 0; m < CPL; m++) f[m] = seg_at(tv[u], m);
+            lds_put<CPL>(smem + (v / LPR) * K + (v % LPR) * CPL, f);
+        }
+    }
+}
+
+// Stage x rows [w0, w0 + nrows), all K columns, in LDS as raw segments (the
+// near window): row i at word i * K * sizeof(XT) / 4, the lane's columns c0..
+// at word c0 * sizeof(XT) / 4 of it -- half the bytes for 2-byte x, and a read
+// gives the same segment a gather would
+template <int K, typename XT, int CPL, bool MASK>
+__device__ __forceinline__ void stage_seg_rows(const XT *__restrict__ X, int ldx, int N, int w0, int nrows, int kv,
+                                               uint32_t *smw)
+{
+    constexpr int LPR = K / CPL;
+    constexpr int RW = K * (int)sizeof(XT) / 4, SW = CPL * (int)sizeof(XT) / 4;
+    const int nw = nrows * LPR;
+    const int tid = threadIdx.x;
+    for (int b = 0; b < nw; b += kWG * 4) {
+        XSeg<XT, CPL> tv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int v = min(b + u * kWG + tid, nw - 1);
+            const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
+            if constexpr (MASK)
+                load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
+            else load_seg<XT, CPL>(xp, tv[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int v = min(b + u * kWG + tid, nw - 1);
+            st_words(smw + (v / LPR) * RW + (v % LPR) * SW, tv[u].w);
         }
     }
 }
@@ -1043,7 +1137,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 }
                 for (int n = sh.ob; n < sh.oe; n++) {
                     const int o = cload(A.offs, n);
-                    float xv[RI][CPL];
+                    XSeg<XT, CPL> xs[RI];
                     bool in[RI];
 #pragma unroll
                     for (int i = 0; i < RI; i++) {
@@ -1051,8 +1145,8 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                         const int c = r - sh.xs + sh.ys + o;
                         in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
                         if (in[i]) {
-                            if constexpr (MASK) load_qm<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xv[i]);
-                            else load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[i]);
+                            if constexpr (MASK) load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[i]);
+                            else load_seg<XT, CPL>(X + (size_t)c * ldx + c0, xs[i]);
                         }
                     }
 #pragma unroll
@@ -1061,8 +1155,10 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                             const int j = n - sh.ob;
                             const uint32_t w = j < 32 ? plo[i] : phi[i];
 #pragma unroll
-                            for (int m = 0; m < CPL; m++)
-                                accum(acc[i * CPL + m], PM ? keep_if(xv[i][m], w, j & 31) : xv[i][m]);
+                            for (int m = 0; m < CPL; m++) {
+                                const float xv = seg_at(xs[i], m);
+                                accum(acc[i * CPL + m], PM ? keep_if(xv, w, j & 31) : xv);
+                            }
                         }
                 }
             };
@@ -1075,10 +1171,13 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
         // kWideHalo) are staged in LDS once and an entry inside them reads x
         // there, not through the texture path
         int wl = 0, wn = 0;
+        constexpr int RW = K * (int)sizeof(XT) / 4;  // words of a near-window row
+        const int c0w = c0 * (int)sizeof(XT) / 4;    // the lane's first word in it
+        uint32_t *smw = (uint32_t *)smem;
         if constexpr (NEAR) {
             wl = max(0, u0 - kWideHalo);
             wn = max(0, min(A.N, u0 + TRW + kWideHalo) - wl);
-            stage_x_rows<K, XT, CPL, MASK>(X, ldx, A.N, wl, wn, kv, smem);
+            stage_seg_rows<K, XT, CPL, MASK>(X, ldx, A.N, wl, wn, kv, smw);
             __syncthreads();
         }
 
@@ -1092,7 +1191,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             for (int u = 0; u < U; u++) nx[u] = rs[i] + u < re[i] ? A.cols[rs[i] + u] : 1;
             for (int j0 = rs[i]; j0 < re[i]; j0 += U) {
                 int v[U];
-                float xv[U][CPL];
+                XSeg<XT, CPL> xs[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) v[u] = nx[u];
                 if (j0 + U < re[i])
@@ -1102,16 +1201,18 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 for (int u = 0; u < U; u++)
                     if (j0 + u < re[i]) {
                         const int c = v[u] >> 1;
-                        if (NEAR && (unsigned)(c - wl) < (unsigned)wn) lds_get<CPL>(smem + (c - wl) * K + c0, xv[u]);
-                        else if constexpr (MASK) load_qm<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xv[u]);
-                        else load_q<XT, CPL>(X + (size_t)c * ldx + c0, xv[u]);
+                        if (NEAR && (unsigned)(c - wl) < (unsigned)wn) ld_words(smw + (c - wl) * RW + c0w, xs[u].w);
+                        else if constexpr (MASK) load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[u]);
+                        else load_seg<XT, CPL>(X + (size_t)c * ldx + c0, xs[u]);
                     }
 #pragma unroll
                 for (int u = 0; u < U; u++)
                     if (j0 + u < re[i])
 #pragma unroll
-                        for (int m = 0; m < CPL; m++)
-                            accum(acc[i * CPL + m], (v[u] & 1) ? xv[u][m] : -xv[u][m]);
+                        for (int m = 0; m < CPL; m++) {
+                            const float xv = seg_at(xs[u], m);
+                            accum(acc[i * CPL + m], (v[u] & 1) ? xv : -xv);
+                        }
             }
         }
 
