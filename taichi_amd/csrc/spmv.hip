@@ -48,10 +48,10 @@ tc_config default_config(int K, int esize, long long remainder, long long diag_t
                                   : (kWG * 2) / K;
     c.lds_values = 4096;
     c.grid = 0;
-    // k = 1: one row per lane for FP32 x (quad rows measured no faster on
-    // R-MAT), quad rows for 2-byte x (2048^2 stencil, BF16: 34.7 against
-    // 40.1 us); the autotuner tries both
-    c.flags = K == 1 && esize == 4 ? kQuadOff : 0;
+    // k = 1: one row per lane (quad rows measured no faster on R-MAT, and
+    // slower on the 2048^2 stencil: 35.7 against 28.5 us FP32, 34.9 against
+    // 31.0 us BF16); the autotuner tries both
+    c.flags = K == 1 ? kQuadOff : 0;
     return c;
 }
 

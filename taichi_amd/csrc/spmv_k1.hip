@@ -138,12 +138,31 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
             else if (q < ne) v = *(const int4 *)(A.cols + lb + e0 + q);
             const int vv[4] = {v.x, v.y, v.z, v.w};
             float xv[4];
+            if constexpr (sizeof(XT) == 4) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int col = vv[k] >> 1;
-                xv[k] = 0.0f;
-                if (DIAG && (unsigned)(col - w0) < (unsigned)TRW) xv[k] = s_win[col - w0];
-                else if (q + k < ne) xv[k] = load_x<XT>(X, (size_t)col);
+                for (int k = 0; k < 4; k++) {
+                    const int col = vv[k] >> 1;
+                    xv[k] = 0.0f;
+                    if (DIAG && (unsigned)(col - w0) < (unsigned)TRW) xv[k] = s_win[col - w0];
+                    else if (q + k < ne) xv[k] = load_x<XT>(X, (size_t)col);
+                }
+            } else {
+                // 2-byte x: bits behind the branch, converted after all four
+                // gathers are issued (a conversion inside it serialises them)
+                uint32_t xb[4];
+                bool inw[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int col = vv[k] >> 1;
+                    inw[k] = DIAG && (unsigned)(col - w0) < (unsigned)TRW;
+                    xb[k] = 0u;
+                    if (!inw[k] && q + k < ne) xb[k] = load_xbits<XT>(X, (size_t)col);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    xv[k] = xbits_f<XT>(xb[k]);
+                    if constexpr (DIAG) xv[k] = inw[k] ? s_win[inw[k] ? (vv[k] >> 1) - w0 : 0] : xv[k];
+                }
             }
             if (q < ne)
                 *(float4 *)(s_val + q) = make_float4((vv[0] & 1) ? xv[0] : -xv[0], (vv[1] & 1) ? xv[1] : -xv[1],

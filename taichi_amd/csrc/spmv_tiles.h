@@ -124,6 +124,24 @@ __device__ __forceinline__ float load_x<f16_t>(const f16_t *__restrict__ X, size
     return (float)X[i];
 }
 
+// x[i] as loaded: all 32 bits (4-byte x) or the element's 16 bits in the low
+// half (2-byte x). A gather that sits behind a branch loads bits and converts
+// them (xbits_f) after the batch's last gather is issued: a conversion inside
+// the branch would make each gather wait for the one before it.
+template <typename XT>
+__device__ __forceinline__ uint32_t load_xbits(const XT *__restrict__ X, size_t i)
+{
+    if constexpr (sizeof(XT) == 4) return __float_as_uint(((const float *)X)[i]);
+    else return (uint32_t)((const uint16_t *)X)[i];
+}
+template <typename XT>
+__device__ __forceinline__ float xbits_f(uint32_t b)
+{
+    if constexpr (sizeof(XT) == 4) return __uint_as_float(b);
+    else if constexpr (__is_same(XT, bf16_t)) return __uint_as_float(b << 16);
+    else return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
+}
+
 // Read-only format metadata (group bases, shapes, offsets, long-row tables)
 // is read through the constant address space: uniform indices then become
 // scalar loads, which do not make the vector loads behind them wait.
@@ -283,9 +301,12 @@ __device__ __forceinline__ void stage_entries(const int *__restrict__ cp, int ne
                 v[u] = cp[j];
             }
         }
+        uint32_t xb[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) xb[u] = b + u * kWG < n ? load_xbits<XT>(X, (size_t)(v[u] >> 1) * ldx + cc[u]) : 0u;
         float xv[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) xv[u] = b + u * kWG < n ? load_x<XT>(X, (size_t)(v[u] >> 1) * ldx + cc[u]) : 0.0f;
+        for (int u = 0; u < U; u++) xv[u] = xbits_f<XT>(xb[u]);
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int idx = b + u * kWG + tid;
@@ -679,13 +700,32 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
                     }
                 }
                 float xv[U];
+                if constexpr (sizeof(XT) == 4) {
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    xv[u] = 0.0f;
-                    if (b + u * kWG < ne) {
+                    for (int u = 0; u < U; u++) {
+                        xv[u] = 0.0f;
+                        if (b + u * kWG < ne) {
+                            const int col = vv[u] >> 1;
+                            if (FAST && (unsigned)(col - w0) < (unsigned)TRW) xv[u] = s_win[col - w0];
+                            else xv[u] = load_x<XT>(X, (size_t)col * ldx);
+                        }
+                    }
+                } else {
+                    // 2-byte x: bits behind the branch, the window value and
+                    // the conversion after every gather is issued
+                    uint32_t xb[U];
+                    bool inw[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
                         const int col = vv[u] >> 1;
-                        if (FAST && (unsigned)(col - w0) < (unsigned)TRW) xv[u] = s_win[col - w0];
-                        else xv[u] = load_x<XT>(X, (size_t)col * ldx);
+                        inw[u] = FAST && (unsigned)(col - w0) < (unsigned)TRW;
+                        xb[u] = 0u;
+                        if (b + u * kWG < ne && !inw[u]) xb[u] = load_xbits<XT>(X, (size_t)col * ldx);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        xv[u] = xbits_f<XT>(xb[u]);
+                        if constexpr (FAST) xv[u] = inw[u] ? s_win[inw[u] ? (vv[u] >> 1) - w0 : 0] : xv[u];
                     }
                 }
 #pragma unroll
