@@ -485,6 +485,22 @@ def main():
         torch.cuda.synchronize()
         host.append(time.perf_counter() - t0)
     ms_host = float(np.median(host)) * 1e3
+    # practical stream ceilings on this part, measured live: a device-to-device
+    # copy whose working set exceeds the 256 MB MALL (1 GiB read + 1 GiB
+    # written) and one that stays MALL-resident like the SpMV's (24 MiB + 24
+    # MiB); GB/s counts read + written bytes
+    ceil = {}
+    for name, nelem, reps in (("copy_1gib_gbs", 1 << 28, 5), ("copy_24mib_gbs", 6 << 20, 50)):
+        src = torch.ones(nelem, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        e0.record(stream)
+        for _ in range(reps):
+            dst.copy_(src)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ceil[name] = round(2 * nelem * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+        del src, dst
     if args.calibrate_copy:
         src = torch.ones(1 << 26, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -520,7 +536,7 @@ def main():
                      "traffic_source": "profiles/spmv_rmat22_traffic.txt: rocprofv3 --pmc FETCH_SIZE, "
                                        "then WRITE_SIZE, on this kernel and matrix, copy-calibrated; "
                                        "read from the committed summary, not measured in this run",
-                     "algorithmic_bytes": B},
+                     "algorithmic_bytes": B, "stream_ceilings": ceil},
         "timing": "hipGraph of the K SpMVs, HIP events on its stream",
         "eager_ms_per_step": round(ms_eager, 5),
         "eager_gnnz_per_s": round(nnz / (ms_eager * 1e-3) / 1e9, 3),
