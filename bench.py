@@ -163,6 +163,12 @@ def bench_batched(dev):
             "speedup": round(ms_plain / ms_batch, 3), "batched_equals_plain": bool(same)}
 
 
+def served_ld(k):
+    """row stride of a served request's X and Y: k rounded up to a vector
+    width of the kernels (1, 2, 4, 8)"""
+    return 1 if k == 1 else 2 if k == 2 else 4 if k <= 4 else 8
+
+
 def bench_served(dev, requests=40000, rate=20000.0):
     """64 resident matrices (32 banded, 32 R-MAT); requests pick a matrix
     uniformly, FP32 or BF16 x, k uniform in 1..8; Poisson arrivals at 20k
@@ -183,20 +189,23 @@ def bench_served(dev, requests=40000, rate=20000.0):
             X = torch.from_numpy(T.gen_vector(n * 8, 0, 7000 + i).reshape(n, 8)).to(dev).to(dt)
             keep.append(X)
             for k in range(1, 9):
-                items.append((mid, X.data_ptr(), 8, code, k))
+                # a request's x holds its own k columns, rows padded to the
+                # kernels' vector width (the first n * ld elements of X)
+                items.append((mid, X.data_ptr(), served_ld(k), code, k))
     ybufs = [torch.empty(nmax, 8, device=dev) for _ in range(512)]
+    LDY = 0  # each request's y rows have the stride of its x rows
     st = srv.stats()
     log(f"config 5: {len(mix)} matrices resident ({st['resident_bytes'] / 1e9:.2f} GB), "
         f"built in {time.time() - t0:.1f}s")
-    srv.run_load(items, ybufs, 8, count=2000, rate=0.0, concurrency=256, seed=1)  # warm-up
-    poisson = srv.run_load(items, ybufs, 8, count=requests, rate=rate, seed=2)
+    srv.run_load(items, ybufs, LDY, count=2000, rate=0.0, concurrency=256, seed=1)  # warm-up
+    poisson = srv.run_load(items, ybufs, LDY, count=requests, rate=rate, seed=2)
     # CLOCK_MONOTONIC windows of the closed loop and the plain loop, the clock
     # a kernel trace is stamped with, to relate device busy time to each
     torch.cuda.synchronize()
     w0 = time.monotonic_ns()
-    closed = srv.run_load(items, ybufs, 8, count=requests, rate=0.0, concurrency=256, seed=3)
+    closed = srv.run_load(items, ybufs, LDY, count=requests, rate=0.0, concurrency=256, seed=3)
     w1 = time.monotonic_ns()
-    plain = srv.run_plain(items, ybufs, 8, count=max(2000, requests // 10), seed=4)
+    plain = srv.run_plain(items, ybufs, LDY, count=max(2000, requests // 10), seed=4)
     w2 = time.monotonic_ns()
     log(f"served windows (monotonic ns): closed {w0} {w1} plain {w1} {w2}")
     srv.close()
@@ -206,6 +215,8 @@ def bench_served(dev, requests=40000, rate=20000.0):
                                             "mean_batch")} | {"completed": r["completed"]}
     return {"workload": "64 resident matrices (32 banded w U{3..64} density 0.7, 32 R-MAT ef 16), "
                         "n U[2^16,2^20], x FP32 or BF16, k U{1..8}",
+            "buffers": "per request: X n x ld and Y n x ld, row-major, ld = k rounded up to 1, 2, 4 "
+                       "or 8 (the kernels' vector widths)",
             "resident_bytes": st["resident_bytes"],
             "poisson_20k": pick(poisson), "saturation": pick(closed), "plain": pick(plain),
             "served_vs_plain_req_per_s": round(closed["req_per_s"] / plain["req_per_s"], 3)}

@@ -307,7 +307,8 @@ int tc_server_get_stats(tc_server *s, tc_server_stats *out);
 
 /* load generation over resident matrices: each request picks a template
    uniformly (seeded) and writes y into ybufs[i % nybufs] (each large enough
-   for any request, row stride ldy) */
+   for any request, row stride ldy; ldy = 0 gives each request's y the row
+   stride of its own x, ldx) */
 typedef struct tc_load_item {
     int matrix_id;
     const void *X;

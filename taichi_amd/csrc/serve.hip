@@ -438,7 +438,8 @@ try {
             while (now_ns() < t_arr) std::this_thread::yield();
         }
         const tc_load_item &it = sp->items[tc::bounded(tc::rand_at(key, 2 * (uint64_t)i + 1), (uint64_t)sp->nitems)];
-        rc = submit_at(S, it.matrix_id, it.X, it.ldx, it.dtype, sp->ybufs[i % sp->nybufs], sp->ldy, it.k, t_arr,
+        rc = submit_at(S, it.matrix_id, it.X, it.ldx, it.dtype, sp->ybufs[i % sp->nybufs], sp->ldy > 0 ? sp->ldy : it.ldx,
+                       it.k, t_arr,
                        &tickets[(size_t)i]);
     }
     for (long long i = waited; i < n && tickets[(size_t)i]; i++) {
@@ -484,7 +485,7 @@ try {
             A = m->second.A;
         }
         const int64_t ta = now_ns();
-        rc = tc_spmv(A, it.X, it.ldx, it.dtype, sp->ybufs[i % sp->nybufs], sp->ldy, it.k, s);
+        rc = tc_spmv(A, it.X, it.ldx, it.dtype, sp->ybufs[i % sp->nybufs], sp->ldy > 0 ? sp->ldy : it.ldx, it.k, s);
         if (rc == TC_OK && hipStreamSynchronize(s) != hipSuccess) rc = TC_ERR_HIP;
         lat_us.push_back((double)(now_ns() - ta) * 1e-3);
     }

@@ -73,6 +73,26 @@ __device__ __forceinline__ void st_nt4(float *p, float a, float b, float c, floa
     __builtin_nontemporal_store(v, (f32x4 *)p);
 }
 
+// 1, 2 or 4 floats: non-temporal (NT) or plain stores
+template <bool NT>
+__device__ __forceinline__ void st1(float *p, float a)
+{
+    if constexpr (NT) st_nt(p, a);
+    else *p = a;
+}
+template <bool NT>
+__device__ __forceinline__ void st2(float *p, float a, float b)
+{
+    if constexpr (NT) st_nt2(p, a, b);
+    else *(float2 *)p = make_float2(a, b);
+}
+template <bool NT>
+__device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
+{
+    if constexpr (NT) st_nt4(p, a, b, c, d);
+    else *(float4 *)p = make_float4(a, b, c, d);
+}
+
 // Offset (in floats) of an 8-byte-aligned array placed after capv floats of LDS
 __host__ __device__ constexpr int lsum_at(int capv) { return (capv + 1) & ~1; }
 
@@ -187,8 +207,15 @@ __device__ __forceinline__ float keep_if(float v, uint32_t w, int j)
     return __int_as_float(__float_as_int(v) & __builtin_amdgcn_sbfe((int)w, j, 1));
 }
 
-// LDS index idx if bit j of w is set, else 0 (where a run of +0.0 is kept)
-__device__ __forceinline__ int keep_idx(int idx, uint32_t w, int j) { return idx & __builtin_amdgcn_sbfe((int)w, j, 1); }
+// LDS index idx if bit j of w is set, else idx mod 64: a run of 64 (+ the
+// read's width) floats of +0.0 is kept at index 0, and reading it at idx mod
+// 64 touches the same LDS banks as reading idx would, so the lanes of an
+// absent term never collide with their neighbours' reads (a single shared
+// zero slot measured 2-4 M bank-conflict cycles a launch on wide bands)
+__device__ __forceinline__ int keep_idx(int idx, uint32_t w, int j)
+{
+    return idx & (__builtin_amdgcn_sbfe((int)w, j, 1) | 63);
+}
 
 // start of row i's short entries (i in [0, M]); row i ends at row_start(i+1)
 __device__ __forceinline__ int row_start(const MatView &A, int i)
@@ -376,15 +403,35 @@ __device__ __forceinline__ void diag_one_shape_t(const MatView &A, const XT *__r
         for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
         if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
             // DU offsets per step: their scalar loads, then all DU * P LDS
-            // reads, then the adds (each pair in offset order)
-            constexpr int DU = 4;
+            // reads, then the adds (each pair in offset order). One column a
+            // pair (K = 1) keeps 16 offsets' reads in flight: each is one
+            // register, and the loop is bound by LDS latency otherwise.
+            constexpr int DU = K == 1 ? 16 : 4;
             for (int seg = 0; seg < nseg; seg++) {
                 const int nb = sh.ob + 32 * seg, ne = PM ? min(sh.oe, nb + 32) : sh.oe;
                 uint32_t w[P];
 #pragma unroll
                 for (int p = 0; p < P; p++) w[p] = seg ? phi[p] : plo[p];
                 int n = nb;
-                for (; n + DU <= ne; n += DU) {
+                if constexpr (DU > 4) {
+                    for (; n + DU <= ne; n += DU) {
+                        int o[DU];
+#pragma unroll
+                        for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * K;
+                        float v[DU][P];
+#pragma unroll
+                        for (int u = 0; u < DU; u++)
+#pragma unroll
+                            for (int p = 0; p < P; p++) v[u][p] = s_val[li[p] + o[u]];
+#pragma unroll
+                        for (int u = 0; u < DU; u++)
+#pragma unroll
+                            for (int p = 0; p < P; p++)
+                                accum(acc[p], PM ? keep_if(v[u][p], w[p], n - nb + u) : v[u][p]);
+                    }
+                }
+                for (; n + 4 <= ne; n += 4) {
+                    constexpr int DU = 4;
                     int o[DU];
 #pragma unroll
                     for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * K;
@@ -820,10 +867,10 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
     }
 }
 
-// 16 bytes of zeros, the in-bounds stand-in for a vector load that would run
+// 32 bytes of zeros, the in-bounds stand-in for a vector load that would run
 // past the end of X (global, not constant, memory: a select between it and an
 // x address stays a global load)
-static __device__ __align__(16) uint32_t kXPad[4] = {0u, 0u, 0u, 0u};
+static __device__ __align__(16) uint32_t kXPad[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 
 // A raw x row segment of Q elements as loaded: Q words for 4-byte x, Q/2
 // words (two elements each) for 2-byte x. Gathers fill segments and the adds
@@ -835,12 +882,16 @@ struct XSeg {
     uint32_t w[NW];
 };
 
-// NW consecutive words (global or LDS): one 4-, 8- or 16-byte access
+// NW consecutive words (global or LDS): one 4-, 8- or 16-byte access, or
+// two 16-byte ones
 template <int NW>
 __device__ __forceinline__ void ld_words(const void *p, uint32_t (&w)[NW])
 {
-    static_assert(NW == 1 || NW == 2 || NW == 4, "x segments are 4, 8 or 16 bytes");
-    if constexpr (NW == 4) {
+    static_assert(NW == 1 || NW == 2 || NW == 4 || NW == 8, "x segments are 4, 8, 16 or 32 bytes");
+    if constexpr (NW == 8) {
+        const uint4 a = ((const uint4 *)p)[0], b = ((const uint4 *)p)[1];
+        w[0] = a.x, w[1] = a.y, w[2] = a.z, w[3] = a.w, w[4] = b.x, w[5] = b.y, w[6] = b.z, w[7] = b.w;
+    } else if constexpr (NW == 4) {
         const uint4 v = *(const uint4 *)p;
         w[0] = v.x, w[1] = v.y, w[2] = v.z, w[3] = v.w;
     } else if constexpr (NW == 2) {
@@ -853,8 +904,11 @@ __device__ __forceinline__ void ld_words(const void *p, uint32_t (&w)[NW])
 template <int NW>
 __device__ __forceinline__ void st_words(void *p, const uint32_t (&w)[NW])
 {
-    static_assert(NW == 1 || NW == 2 || NW == 4, "x segments are 4, 8 or 16 bytes");
-    if constexpr (NW == 4) *(uint4 *)p = make_uint4(w[0], w[1], w[2], w[3]);
+    static_assert(NW == 1 || NW == 2 || NW == 4 || NW == 8, "x segments are 4, 8, 16 or 32 bytes");
+    if constexpr (NW == 8) {
+        ((uint4 *)p)[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        ((uint4 *)p)[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    } else if constexpr (NW == 4) *(uint4 *)p = make_uint4(w[0], w[1], w[2], w[3]);
     else if constexpr (NW == 2) *(uint2 *)p = make_uint2(w[0], w[1]);
     else *(uint32_t *)p = w[0];
 }
@@ -878,55 +932,71 @@ __device__ __forceinline__ void load_seg(const XT *__restrict__ p, XSeg<XT, Q> &
 // Segment of a masked chunk. The whole vector is loaded: it lies inside the
 // row's ldx span (ldx is a multiple of Q and the lane's first column is below
 // kv <= ldx), except in the last row of X, which may end at column kv --
-// there (last, nv < Q) the vector comes from kXPad and only the first nv
-// elements are read, one by one; the others are 0. Elsewhere the elements at
-// or past nv are whatever X holds there: they feed only columns that are
-// never stored. The vector load is unconditional (an address select); only
-// the rare tail is a branch.
+// there (the returned tail: last, nv < Q) the vector comes from kXPad, and
+// seg_m_tail later puts the first nv elements in, read one by one (the
+// others stay 0). Elsewhere the elements at or past nv are whatever X holds
+// there: they feed only columns that are never stored. The vector load is
+// unconditional (an address select). Callers issue a whole batch of loads
+// before the rare tail fix-ups: a fix-up writes the segment's registers, so
+// it waits for the loads, and one inside the batch would serialise them.
 template <typename XT, int Q>
-__device__ __forceinline__ void load_seg_m(const XT *__restrict__ p, int nv, bool last, XSeg<XT, Q> &s)
+__device__ __forceinline__ bool load_seg_m(const XT *__restrict__ p, int nv, bool last, XSeg<XT, Q> &s)
 {
     const bool tail = last && nv < Q;
     ld_words(tail ? (const void *)kXPad : (const void *)p, s.w);
-    if (tail) {
+    return tail;
+}
+template <typename XT, int Q>
+__device__ __forceinline__ void seg_m_tail(const XT *__restrict__ p, int nv, XSeg<XT, Q> &s)
+{
+    uint32_t w[XSeg<XT, Q>::NW];
 #pragma unroll
-        for (int m = 0; m < Q; m++)
-            if (m < nv) {
-                if constexpr (sizeof(XT) == 4) s.w[m] = ((const uint32_t *)p)[m];
-                else s.w[m >> 1] |= (uint32_t)((const uint16_t *)p)[m] << (16 * (m & 1));
-            }
-    }
+    for (int i = 0; i < XSeg<XT, Q>::NW; i++) w[i] = 0u;
+#pragma unroll
+    for (int m = 0; m < Q; m++)
+        if (m < nv) {
+            if constexpr (sizeof(XT) == 4) w[m] = ((const uint32_t *)p)[m];
+            else w[m >> 1] |= (uint32_t)((const uint16_t *)p)[m] << (16 * (m & 1));
+        }
+#pragma unroll
+    for (int i = 0; i < XSeg<XT, Q>::NW; i++) s.w[i] = w[i];
 }
 
 // The first nv (0 < nv < Q) of Q values to yp (Q-aligned) in aligned pieces
 // of 4, 2 and 1: at most three stores, nothing past column nv. Indices are
 // constants in every branch, so f stays in registers.
-template <int Q>
+template <int Q, bool NT>
 __device__ __forceinline__ void store_first(float *yp, const float (&f)[Q], int nv)
 {
     if constexpr (Q >= 8) {
         if (nv >= 4) {
-            st_nt4(yp, f[0], f[1], f[2], f[3]);
-            if (nv == 5) st_nt(yp + 4, f[4]);
-            else if (nv >= 6) st_nt2(yp + 4, f[4], f[5]);
-            if (nv == 7) st_nt(yp + 6, f[6]);
+            st4<NT>(yp, f[0], f[1], f[2], f[3]);
+            if (nv == 5) st1<NT>(yp + 4, f[4]);
+            else if (nv >= 6) st2<NT>(yp + 4, f[4], f[5]);
+            if (nv == 7) st1<NT>(yp + 6, f[6]);
             return;
         }
     }
     if constexpr (Q >= 4) {
         if (nv >= 2) {
-            st_nt2(yp, f[0], f[1]);
-            if (nv == 3) st_nt(yp + 2, f[2]);
+            st2<NT>(yp, f[0], f[1]);
+            if (nv == 3) st1<NT>(yp + 2, f[2]);
             return;
         }
     }
-    st_nt(yp, f[0]);
+    st1<NT>(yp, f[0]);
 }
 
 // Columns a lane owns in the wide-rows tiles: 4 (a 16-byte FP32 vector, an
 // 8-byte BF16/FP16 one), or 8 for 2-byte x when K >= 8 (a 16-byte vector: half
-// the lanes per row, half the vector memory instructions)
-__host__ __device__ constexpr int wide_cpl(int K, int esize) { return esize == 2 && K >= 8 ? 8 : (K < 4 ? K : 4); }
+// the lanes per row, half the vector memory instructions) and for FP32 x at
+// K = 8 (two 16-byte vectors: one lane a row, so the per-offset address and
+// presence mask of a dense diagonal are paid once a row, not twice; the
+// diagonal loop of wide bands is bound by VALU issue)
+__host__ __device__ constexpr int wide_cpl(int K, int esize)
+{
+    return (esize == 2 && K >= 8) || K == 8 ? 8 : (K < 4 ? K : 4);
+}
 
 // ---- k > 1: wide rows ----
 // A row belongs to a group of LPR = K/CPL lanes; lane q of the group owns the
@@ -947,14 +1017,14 @@ __host__ __device__ constexpr int wide_cpl(int K, int esize) { return esize == 2
 // floats, K <= 8) and for the long-row sums (TRW * K floats); the launch
 // sizes LDS by wide_lds_bytes.
 constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
-constexpr int kWideZero = 8;   // floats of +0.0 before a shape's x window (absent terms read them)
+constexpr int kWideZero = 64;  // floats of +0.0 before a shape's x window (absent terms read them, keep_idx)
 
 // LDS bytes of a wide-rows tile of trw rows and K columns: the shape window
 // (or, near, the rows around the tile the remainder reads) and the long-row
 // sums (lng)
 __host__ __device__ constexpr size_t wide_lds_bytes(int K, int trw, bool near, bool lng)
 {
-    const size_t win = ((size_t)(trw + (near && K <= 8 ? 2 : 1) * kWideHalo) * K + kWideZero) * 4;
+    const size_t win = ((size_t)(trw + (near && K <= 8 ? 2 : 1) * kWideHalo) * K + 3 * kWideZero) * 4;
     const size_t lsum = lng ? (size_t)trw * K * sizeof(acc_t) : 0;
     return win > lsum ? win : lsum;
 }
@@ -963,22 +1033,33 @@ __host__ __device__ constexpr size_t wide_lds_bytes(int K, int trw, bool near, b
 // w0) * K + col], CPL-wide vectors, four loads in flight per lane. Lanes past
 // the window repeat its last vector (same address, same value), so neither
 // the loads nor the LDS stores sit behind a branch.
-template <int K, typename XT, int CPL, bool MASK>
+template <int K, typename XT, int CPL, bool MASK, bool PLANAR = false>
 __device__ __forceinline__ void stage_x_rows(const XT *__restrict__ X, int ldx, int N, int w0, int nrows, int kv,
-                                             float *smem)
+                                             float *smem, int ps = 0)
 {
     constexpr int LPR = K / CPL;
     const int nw = nrows * LPR;  // CPL-wide vectors of the window
     const int tid = threadIdx.x;
     for (int b = 0; b < nw; b += kWG * 4) {
         XSeg<XT, CPL> tv[4];
+        bool tl[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int v = min(b + u * kWG + tid, nw - 1);
             const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
+            tl[u] = false;
             if constexpr (MASK)
-                load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
+                tl[u] = load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
             else load_seg<XT, CPL>(xp, tv[u]);
+        }
+        if constexpr (MASK) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (tl[u]) {
+                    const int v = min(b + u * kWG + tid, nw - 1);
+                    seg_m_tail<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL,
+                                        min(max(kv - (v % LPR) * CPL, 0), CPL), tv[u]);
+                }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -986,7 +1067,15 @@ __device__ __forceinline__ void stage_x_rows(const XT *__restrict__ X, int ldx, 
             float f[CPL];
 #pragma unroll
             for (int m = 0; m < CPL; m++) f[m] = seg_at(tv[u], m);
-            lds_put<CPL>(smem + (v / LPR) * K + (v % LPR) * CPL, f);
+            if constexpr (PLANAR) {
+                float lo[4], hi[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) lo[m] = f[m], hi[m] = f[4 + m];
+                lds_put<4>(smem + v * 4, lo);
+                lds_put<4>(smem + ps + v * 4, hi);
+            } else {
+                lds_put<CPL>(smem + (v / LPR) * K + (v % LPR) * CPL, f);
+            }
         }
     }
 }
@@ -1005,13 +1094,24 @@ __device__ __forceinline__ void stage_seg_rows(const XT *__restrict__ X, int ldx
     const int tid = threadIdx.x;
     for (int b = 0; b < nw; b += kWG * 4) {
         XSeg<XT, CPL> tv[4];
+        bool tl[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int v = min(b + u * kWG + tid, nw - 1);
             const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
+            tl[u] = false;
             if constexpr (MASK)
-                load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
+                tl[u] = load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
             else load_seg<XT, CPL>(xp, tv[u]);
+        }
+        if constexpr (MASK) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (tl[u]) {
+                    const int v = min(b + u * kWG + tid, nw - 1);
+                    seg_m_tail<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL,
+                                        min(max(kv - (v % LPR) * CPL, 0), CPL), tv[u]);
+                }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -1032,6 +1132,10 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
     constexpr int TRW = GPW * RI;   // rows per tile
     constexpr int P = CPL * RI;     // (row, column) pairs per lane
     constexpr int U = 4;            // remainder entries in flight per lane
+    // a shape's LDS window: two planes of 4 columns when one lane owns a
+    // row's 8 columns, else rows of K columns; RS floats a window row
+    constexpr bool PLANAR = CPL == 8 && LPR == 1;
+    constexpr int RS = PLANAR ? 4 : K;
     const int tid = threadIdx.x;
     const int g = tid / LPR, c0 = (tid % LPR) * CPL;
     const int nv = MASK ? min(max(kv - c0, 0), CPL) : CPL;  // this lane's columns that exist
@@ -1084,12 +1188,31 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 }
                 if (sh.oe - sh.ob >= 2 && w1 > w0 && w1 - w0 <= TRW + kWideHalo) {
                     // the window starts past kWideZero floats of +0.0: a term
-                    // a bitmap marks absent reads them (its LDS index masked
-                    // to 0), so one AND per row and offset replaces one per
-                    // column
-                    stage_x_rows<K, XT, CPL, MASK>(X, ldx, A.N, w0, w1 - w0, kv, smem + kWideZero);
-                    if (tid < kWideZero) smem[tid] = 0.0f;
+                    // a bitmap marks absent reads them (keep_idx), so one
+                    // mask per row and offset replaces one per column. With 8
+                    // columns a lane (one lane a row) the window is two
+                    // planes of 4 columns, ps floats apart, each with its
+                    // zero run: every read of a wavefront is then 16
+                    // contiguous bytes a lane (32-byte rows measured heavy
+                    // bank conflicts)
+                    const int ps = PLANAR ? (kWideZero + (w1 - w0) * 4 + 63) & ~63 : 0;
+                    stage_x_rows<K, XT, CPL, MASK, PLANAR>(X, ldx, A.N, w0, w1 - w0, kv, smem + kWideZero, ps);
+                    if (tid < kWideZero) {
+                        smem[tid] = 0.0f;
+                        if (PLANAR) smem[ps + tid] = 0.0f;
+                    }
                     __syncthreads();
+                    auto win_get = [&](int idx, float (&q)[CPL]) {
+                        if constexpr (PLANAR) {
+                            float lo[4], hi[4];
+                            lds_get<4>(smem + idx, lo);
+                            lds_get<4>(smem + ps + idx, hi);
+#pragma unroll
+                            for (int m = 0; m < 4; m++) q[m] = lo[m], q[4 + m] = hi[m];
+                        } else {
+                            lds_get<CPL>(smem + idx, q);
+                        }
+                    };
                     // every row of the wavefront inside the shape with all its
                     // columns: DU offsets per step, their scalar loads, then
                     // the DU * RI LDS vectors, then the adds in offset order
@@ -1105,7 +1228,8 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                         const int nseg = PM ? (sh.oe - sh.ob + 31) >> 5 : 1;
                         int base[RI];
 #pragma unroll
-                        for (int i = 0; i < RI; i++) base[i] = kWideZero + (u0 + GPW * i + g + shift - w0) * K + c0;
+                        for (int i = 0; i < RI; i++)
+                            base[i] = kWideZero + (u0 + GPW * i + g + shift - w0) * RS + (PLANAR ? 0 : c0);
                         for (int seg = 0; seg < nseg; seg++) {
                             const int nb = sh.ob + 32 * seg, ne = PM ? min(sh.oe, nb + 32) : sh.oe;
                             uint32_t w[RI];
@@ -1119,18 +1243,41 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                             // RI * CPL x values are live and occupancy holds.
                             // The rest run one offset at a time.
                             int n = nb;
+                            // up to 8 columns with at most 4 x values a row
+                            // per lane: the group's DU * RI LDS vectors are
+                            // issued together, then the adds in offset order
+                            // (16 registers in flight, occupancy holds)
+                            constexpr bool kAll = K <= 8 && RI * CPL <= 4;
+                            if constexpr (kAll) {
+                                for (; n + DU <= ne; n += DU) {
+                                    int o[DU];
+#pragma unroll
+                                    for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * RS;
+                                    float q[DU][RI][CPL];
+#pragma unroll
+                                    for (int u = 0; u < DU; u++)
+#pragma unroll
+                                        for (int i = 0; i < RI; i++)
+                                            win_get(PM ? keep_idx(base[i] + o[u], w[i], n - nb + u) : base[i] + o[u],
+                                                    q[u][i]);
+#pragma unroll
+                                    for (int u = 0; u < DU; u++)
+#pragma unroll
+                                        for (int i = 0; i < RI; i++)
+#pragma unroll
+                                            for (int m = 0; m < CPL; m++) accum(acc[i * CPL + m], q[u][i][m]);
+                                }
+                            }
                             for (; n + DU <= ne; n += DU) {
                                 int o[DU];
 #pragma unroll
-                                for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * K;
+                                for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * RS;
 #pragma unroll
                                 for (int u = 0; u < DU; u++) {
                                     float q[RI][CPL];
 #pragma unroll
                                     for (int i = 0; i < RI; i++)
-                                        lds_get<CPL>(smem + (PM ? keep_idx(base[i] + o[u], w[i], n - nb + u)
-                                                                : base[i] + o[u]),
-                                                     q[i]);
+                                        win_get(PM ? keep_idx(base[i] + o[u], w[i], n - nb + u) : base[i] + o[u], q[i]);
 #pragma unroll
                                     for (int i = 0; i < RI; i++)
 #pragma unroll
@@ -1139,11 +1286,11 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                                 }
                             }
                             for (; n < ne; n++) {
-                                const int o = cload(A.offs, n) * K;
+                                const int o = cload(A.offs, n) * RS;
                                 float q[RI][CPL];
 #pragma unroll
                                 for (int i = 0; i < RI; i++)
-                                    lds_get<CPL>(smem + (PM ? keep_idx(base[i] + o, w[i], n - nb) : base[i] + o), q[i]);
+                                    win_get(PM ? keep_idx(base[i] + o, w[i], n - nb) : base[i] + o, q[i]);
 #pragma unroll
                                 for (int i = 0; i < RI; i++)
 #pragma unroll
@@ -1163,7 +1310,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                                     const int j = n - sh.ob;
                                     const uint32_t w = j < 32 ? plo[i] : phi[i];
                                     float q[CPL];
-                                    lds_get<CPL>(smem + kWideZero + (c - w0) * K + c0, q);
+                                    win_get(kWideZero + (c - w0) * RS + (PLANAR ? 0 : c0), q);
 #pragma unroll
                                     for (int m = 0; m < CPL; m++)
                                         accum(acc[i * CPL + m], PM ? keep_if(q[m], w, j & 31) : q[m]);
@@ -1178,16 +1325,23 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 for (int n = sh.ob; n < sh.oe; n++) {
                     const int o = cload(A.offs, n);
                     XSeg<XT, CPL> xs[RI];
-                    bool in[RI];
+                    bool in[RI], tl[RI];
 #pragma unroll
                     for (int i = 0; i < RI; i++) {
                         const int r = u0 + GPW * i + g;
                         const int c = r - sh.xs + sh.ys + o;
                         in[i] = r < A.M && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + side;
+                        tl[i] = false;
                         if (in[i]) {
-                            if constexpr (MASK) load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[i]);
+                            if constexpr (MASK)
+                                tl[i] = load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[i]);
                             else load_seg<XT, CPL>(X + (size_t)c * ldx + c0, xs[i]);
                         }
+                    }
+                    if constexpr (MASK) {
+#pragma unroll
+                        for (int i = 0; i < RI; i++)
+                            if (tl[i]) seg_m_tail<XT, CPL>(X + (size_t)(A.N - 1) * ldx + c0, nv, xs[i]);
                     }
 #pragma unroll
                     for (int i = 0; i < RI; i++)
@@ -1237,14 +1391,23 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 if (j0 + U < re[i])
 #pragma unroll
                     for (int u = 0; u < U; u++) nx[u] = j0 + U + u < re[i] ? A.cols[j0 + U + u] : 1;
+                bool tl[U];
 #pragma unroll
-                for (int u = 0; u < U; u++)
+                for (int u = 0; u < U; u++) {
+                    tl[u] = false;
                     if (j0 + u < re[i]) {
                         const int c = v[u] >> 1;
                         if (NEAR && (unsigned)(c - wl) < (unsigned)wn) ld_words(smw + (c - wl) * RW + c0w, xs[u].w);
-                        else if constexpr (MASK) load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[u]);
+                        else if constexpr (MASK)
+                            tl[u] = load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[u]);
                         else load_seg<XT, CPL>(X + (size_t)c * ldx + c0, xs[u]);
                     }
+                }
+                if constexpr (MASK) {
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        if (tl[u]) seg_m_tail<XT, CPL>(X + (size_t)(A.N - 1) * ldx + c0, nv, xs[u]);
+                }
 #pragma unroll
                 for (int u = 0; u < U; u++)
                     if (j0 + u < re[i])
@@ -1267,20 +1430,25 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
             const int r = u0 + GPW * i + g;
             if (r >= A.M) continue;
             float *yp = Y + (size_t)r * ldy + c0;
+            // a masked 8-wide row (5..7 of its 8 columns stored) leaves
+            // through plain stores, every lane of it: its bytes then meet in
+            // L2 and leave as one line, where non-temporal partial stores
+            // measured up to 1.8x slower; everything else streams past L2
+            constexpr bool NT = !(MASK && K == 8);
             if (MASK && nv < CPL) {
                 float f[CPL];
 #pragma unroll
                 for (int m = 0; m < CPL; m++) f[m] = (float)acc[i * CPL + m];
-                store_first<CPL>(yp, f, nv);
+                store_first<CPL, NT>(yp, f, nv);
                 continue;
             }
             if constexpr (CPL == 2) {
-                st_nt2(yp, (float)acc[i * 2], (float)acc[i * 2 + 1]);
+                st2<NT>(yp, (float)acc[i * 2], (float)acc[i * 2 + 1]);
             } else {
 #pragma unroll
                 for (int h = 0; h < CPL; h += 4)
-                    st_nt4(yp + h, (float)acc[i * CPL + h], (float)acc[i * CPL + h + 1], (float)acc[i * CPL + h + 2],
-                           (float)acc[i * CPL + h + 3]);
+                    st4<NT>(yp + h, (float)acc[i * CPL + h], (float)acc[i * CPL + h + 1], (float)acc[i * CPL + h + 2],
+                            (float)acc[i * CPL + h + 3]);
             }
         }
     }

@@ -232,7 +232,7 @@ def test_k_columns_wide_equals_staged(case, dev):
         Xi = T.gen_vector(N * k, 1, 50 + k).reshape(N, k)
         Xd = torch.from_numpy(Xi).to(dev)
         outs = []
-        cpl = min(k, 4)
+        cpl = 8 if k == 8 else min(k, 4)  # FP32 columns a lane owns (wide_cpl)
         for rows, flags in [(256 * cpl // k, 0), (512 * cpl // k, 0), (512 // k, 4)]:
             A.set_config(k, T.F32, rows, 4096, 0, flags)
             Y = torch.full((M, k), float("nan"), device=dev)
