@@ -221,13 +221,9 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
             c0 += 8;
         }
         const int rest = k - c0;
-        // 3, 5, 6 or 7 columns run as one masked pass of width 4 or 8, except
-        // 5 or 6 FP32 columns of a matrix with many dense diagonals: there
-        // the masked pass gives a second lane of each row one or two columns,
-        // and 4 + 1 or 4 + 2 measured 7-10 % faster. (For 2-byte x a lane owns
-        // all 8 columns and the masked pass is 9-10 % faster.)
-        const bool split = r.A->st.num_negs > 2 && (rest == 5 || rest == 6) && elem_size(r.dtype) == 4;
-        const int W = rest > 0 && !split ? masked_width(rest) : 0;
+        // 3, 5, 6 or 7 columns run as one masked pass of width 4 or 8 (a
+        // lane owns all 8 columns of a row, for every x type)
+        const int W = rest > 0 ? masked_width(rest) : 0;
         if (W && chunk_aligned(r, W, c0) && !(get_cfg(r.A, W, r.dtype).flags & kNoMask)) {
             f(W == 2 ? 1 : W == 4 ? 2 : 3, W, c0, rest);
             return;
