@@ -91,11 +91,15 @@ int plan_layout(int M, int N, const TaiChi_new *T, int long_threshold, Layout &P
             return tc::fail(TC_ERR_INVALID, "matrix layout: shape %d out of range", s);
         h.omin = 0;
         h.omax = 0;
+        h.seq = 0;
+        h.pad0 = h.pad1 = h.pad2 = 0;
         if (h.oe > h.ob) {
             h.omin = h.omax = T->neg_offsets[h.ob];
+            h.seq = 1;
             for (int n = h.ob; n < h.oe; n++) {
                 h.omin = std::min(h.omin, T->neg_offsets[n]);
                 h.omax = std::max(h.omax, T->neg_offsets[n]);
+                if ((long long)T->neg_offsets[n] != (long long)T->neg_offsets[h.ob] + (n - h.ob)) h.seq = 0;
             }
         }
         h.pm = -1;

@@ -98,9 +98,12 @@ __host__ __device__ constexpr int lsum_at(int capv) { return (capv + 1) & ~1; }
 
 // One shape: rows [xs, xe), offsets offs[ob..oe) (spread omin..omax), term
 // of row r and offset o at column ys + (r - xs) + o; pm = word offset of its
-// presence bitmap in pres, or -1 when every term is present.
+// presence bitmap in pres, or -1 when every term is present; seq = 1 when
+// the offsets are stored as omin, omin + 1, ..., omax (a band: offs[ob + j]
+// = omin + j), else 0.
 struct alignas(16) ShapeDev {
     int xs, xe, ys, ob, oe, omin, omax, pm;
+    int seq, pad0, pad1, pad2;
 };
 
 // Device pointers and sizes of one resident matrix; passed by value.
@@ -183,6 +186,8 @@ __device__ __forceinline__ ShapeDev load_shape(const MatView &A, int s)
     h.omin = cload(p, 5);
     h.omax = cload(p, 6);
     h.pm = cload(p, 7);
+    h.seq = cload(p, 8);
+    h.pad0 = h.pad1 = h.pad2 = 0;
     return h;
 }
 
@@ -834,6 +839,27 @@ __device__ __forceinline__ void lds_put(float *d, const float (&v)[Q])
     }
 }
 
+// LDS byte address of a shared-memory pointer, and Q floats (Q = 2, 4, 8) read
+// at an LDS byte address (constant parts of the address become the
+// instruction's immediate offset)
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ uint32_t lds_addr(const float *p) { return (uint32_t)(uintptr_t)(const lds_float *)p; }
+template <int Q>
+__device__ __forceinline__ void lds_get_at(uint32_t a, float (&o)[Q])
+{
+    static_assert(Q == 2 || Q == 4 || Q == 8, "2, 4 or 8 floats");
+    if constexpr (Q == 2) {
+        const f32x2 v = *(const __attribute__((address_space(3))) f32x2 *)a;
+        o[0] = v[0], o[1] = v[1];
+    } else {
+#pragma unroll
+        for (int h = 0; h < Q; h += 4) {
+            const f32x4 v = *(const __attribute__((address_space(3))) f32x4 *)(a + 4 * h);
+            o[h] = v[0], o[h + 1] = v[1], o[h + 2] = v[2], o[h + 3] = v[3];
+        }
+    }
+}
+
 // x row segment of Q elements as floats: one vector load (Q * sizeof(XT)
 // bytes: 8 or 16 for 4-byte x, 4, 8 or 16 for 2-byte x)
 template <typename XT, int Q>
@@ -1012,19 +1038,28 @@ __host__ __device__ constexpr int wide_cpl(int K, int esize)
 // bounds). A masked chunk runs k = 3, 5, 6, 7 as one pass of width 4 or 8
 // instead of 2 + 1, 4 + 1, 4 + 2, 4 + 2 + 1 passes. Needs kv > K/2 (so a
 // lane's vector stays inside its X row, or X's last row is handled apart).
-// LDS: the x window [(TRW + kWideHalo) * K] floats, reused for the rows
-// around the tile that a near remainder reads ((TRW + 2 kWideHalo) * K
-// floats, K <= 8) and for the long-row sums (TRW * K floats); the launch
-// sizes LDS by wide_lds_bytes.
+// LDS: a shape's x window (a run of kWideZero floats of +0.0, then up to
+// TRW + kWideHalo rows of K columns; for K = 8 two planes of 4 columns,
+// wide_plane floats apart), reused for the rows around the tile that a near
+// remainder reads ((TRW + 2 kWideHalo) * K floats, K <= 8) and for the
+// long-row sums (TRW * K floats); the launch sizes LDS by wide_lds_bytes.
 constexpr int kWideHalo = 64;  // window rows beyond the tile's that LDS holds (offset spread)
-constexpr int kWideZero = 64;  // floats of +0.0 before a shape's x window (absent terms read them, keep_idx)
+// floats of +0.0 before a shape's x window (before each plane): absent terms
+// read them at an index congruent mod 64 to the present term's, and the
+// contiguous-offset loop reads them at that index + j * 4 for offset j < 64
+constexpr int kWideZero = 320;
+
+// distance in floats between the two planes of a K = 8 window (tiles of trw rows)
+__host__ __device__ constexpr int wide_plane(int trw) { return (kWideZero + (trw + kWideHalo) * 4 + 63) & ~63; }
 
 // LDS bytes of a wide-rows tile of trw rows and K columns: the shape window
 // (or, near, the rows around the tile the remainder reads) and the long-row
 // sums (lng)
 __host__ __device__ constexpr size_t wide_lds_bytes(int K, int trw, bool near, bool lng)
 {
-    const size_t win = ((size_t)(trw + (near && K <= 8 ? 2 : 1) * kWideHalo) * K + 3 * kWideZero) * 4;
+    const size_t shape = K == 8 ? (size_t)2 * wide_plane(trw) * 4 : ((size_t)(trw + kWideHalo) * K + kWideZero) * 4;
+    const size_t nearw = near && K <= 8 ? (size_t)(trw + 2 * kWideHalo) * K * 4 : 0;
+    const size_t win = shape > nearw ? shape : nearw;
     const size_t lsum = lng ? (size_t)trw * K * sizeof(acc_t) : 0;
     return win > lsum ? win : lsum;
 }
@@ -1121,6 +1156,18 @@ __device__ __forceinline__ void stage_seg_rows(const XT *__restrict__ X, int ldx
     }
 }
 
+// f(integral_constant<JB>) for JB = 0, DU, 2 DU, ... below min(nd, 64):
+// a loop over groups of DU offsets unrolled at compile time
+template <int DU, int JB = 0, class F>
+__device__ __forceinline__ void offset_groups(int nd, F &&f)
+{
+    if constexpr (JB < 64) {
+        if (JB >= nd) return;
+        f(std::integral_constant<int, JB>{});
+        offset_groups<DU, JB + DU>(nd, f);
+    }
+}
+
 template <int K, typename XT, int RI, bool LONG, bool MASK = false, bool NEAR = false>
 __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict__ X, int ldx, float *__restrict__ Y,
                                           int ldy, int t, float *smem, int kv = K)
@@ -1195,13 +1242,25 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     // zero run: every read of a wavefront is then 16
                     // contiguous bytes a lane (32-byte rows measured heavy
                     // bank conflicts)
-                    const int ps = PLANAR ? (kWideZero + (w1 - w0) * 4 + 63) & ~63 : 0;
+                    constexpr int ps = PLANAR ? wide_plane(TRW) : 0;
                     stage_x_rows<K, XT, CPL, MASK, PLANAR>(X, ldx, A.N, w0, w1 - w0, kv, smem + kWideZero, ps);
-                    if (tid < kWideZero) {
-                        smem[tid] = 0.0f;
-                        if (PLANAR) smem[ps + tid] = 0.0f;
+                    for (int q = tid; q < kWideZero; q += kWG) {
+                        smem[q] = 0.0f;
+                        if (PLANAR) smem[ps + q] = 0.0f;
                     }
                     __syncthreads();
+                    // the same at an LDS byte address
+                    auto win_at = [&](uint32_t a, float (&q)[CPL]) {
+                        if constexpr (PLANAR) {
+                            float lo[4], hi[4];
+                            lds_get_at<4>(a, lo);
+                            lds_get_at<4>(a + ps * 4, hi);
+#pragma unroll
+                            for (int m = 0; m < 4; m++) q[m] = lo[m], q[4 + m] = hi[m];
+                        } else {
+                            lds_get_at<CPL>(a, q);
+                        }
+                    };
                     auto win_get = [&](int idx, float (&q)[CPL]) {
                         if constexpr (PLANAR) {
                             float lo[4], hi[4];
@@ -1223,7 +1282,56 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                         inner = inner && r < A.M && r >= sh.xs && r < sh.xe && r + shift + sh.omin >= sh.ys &&
                                 r + shift + sh.omax < sh.ys + side;
                     }
-                    if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+                    const int nd = sh.oe - sh.ob;
+                    const bool contig = RS <= 4 && nd <= 64 && sh.seq;
+                    if (__builtin_amdgcn_ballot_w64(!inner) == 0 && contig) {
+                        // contiguous offsets omin + j (a band): offset j of a
+                        // row reads its x at b0 + j * RS, or, absent, at
+                        // (b0 mod 64) + j * RS inside the zero run (same
+                        // banks). One bit extract and one multiply-add pick
+                        // the base; j * RS is the LDS instruction's immediate
+                        // offset. Offsets in order, DU reads in flight. The
+                        // pick is one bit-select (v_bitop3) on the
+                        // sign-extended presence bit.
+                        uint32_t mlo[RI], mhi[RI];
+                        uint32_t ap[RI], az[RI];  // LDS byte addresses: present base, zero-run base
+                        const uint32_t klo = nd >= 32 ? ~0u : (1u << nd) - 1u;
+                        const uint32_t khi = nd >= 64 ? ~0u : nd > 32 ? (1u << (nd - 32)) - 1u : 0u;
+#pragma unroll
+                        for (int i = 0; i < RI; i++) {
+                            mlo[i] = (PM ? plo[i] : ~0u) & klo;
+                            mhi[i] = (PM ? phi[i] : ~0u) & khi;
+                            // in bytes: the bank period is 256 bytes
+                            const int b0 = 4 * (kWideZero + (u0 + GPW * i + g + shift + sh.omin - w0) * RS +
+                                                (PLANAR ? 0 : c0));
+                            ap[i] = lds_addr(smem) + (uint32_t)b0;
+                            az[i] = lds_addr(smem) + (uint32_t)(b0 & 255);
+                        }
+                        constexpr int DU = RI * CPL <= 4 ? 8 : 4;
+                        // groups of DU offsets, unrolled at compile time (j
+                        // a constant: immediate bit positions and offsets)
+                        offset_groups<DU>(nd, [&](auto jbc) {
+                            constexpr int jb = decltype(jbc)::value;
+                            float q[DU][RI][CPL];
+#pragma unroll
+                            for (int u = 0; u < DU; u++)
+#pragma unroll
+                                for (int i = 0; i < RI; i++) {
+                                    const int j = jb + u;
+                                    const uint32_t m =
+                                        (uint32_t)__builtin_amdgcn_sbfe((int)(j < 32 ? mlo[i] : mhi[i]), j & 31, 1);
+                                    // m ? ap : az (truth table of s1 ? s0 : s2)
+                                    const uint32_t a = __builtin_amdgcn_bitop3_b32(ap[i], m, az[i], 0xE2);
+                                    win_at(a + j * RS * 4, q[u][i]);
+                                }
+#pragma unroll
+                            for (int u = 0; u < DU; u++)
+#pragma unroll
+                                for (int i = 0; i < RI; i++)
+#pragma unroll
+                                    for (int m = 0; m < CPL; m++) accum(acc[i * CPL + m], q[u][i][m]);
+                        });
+                    } else if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
                         constexpr int DU = 4;
                         const int nseg = PM ? (sh.oe - sh.ob + 31) >> 5 : 1;
                         int base[RI];
