@@ -138,17 +138,16 @@ __device__ __forceinline__ void quad_tile(const MatView &A, const XT *__restrict
             else if (q < ne) v = *(const int4 *)(A.cols + lb + e0 + q);
             const int vv[4] = {v.x, v.y, v.z, v.w};
             float xv[4];
-            if constexpr (sizeof(XT) == 4) {
+            if constexpr (sizeof(XT) == 4 && !DIAG) {
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const int col = vv[k] >> 1;
                     xv[k] = 0.0f;
-                    if (DIAG && (unsigned)(col - w0) < (unsigned)TRW) xv[k] = s_win[col - w0];
-                    else if (q + k < ne) xv[k] = load_x<XT>(X, (size_t)col);
+                    if (q + k < ne) xv[k] = load_x<XT>(X, (size_t)(vv[k] >> 1));
                 }
             } else {
-                // 2-byte x: bits behind the branch, converted after all four
-                // gathers are issued (a conversion inside it serialises them)
+                // bits behind the branch, the window value and the conversion
+                // after all four gathers are issued (a conversion, or a window
+                // read into the gather's register, inside it serialises them)
                 uint32_t xb[4];
                 bool inw[4];
 #pragma unroll

@@ -408,35 +408,17 @@ __device__ __forceinline__ void diag_one_shape_t(const MatView &A, const XT *__r
         for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
         if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
             // DU offsets per step: their scalar loads, then all DU * P LDS
-            // reads, then the adds (each pair in offset order). One column a
-            // pair (K = 1) keeps 16 offsets' reads in flight: each is one
-            // register, and the loop is bound by LDS latency otherwise.
-            constexpr int DU = K == 1 ? 16 : 4;
+            // reads, then the adds (each pair in offset order). (16 offsets
+            // in flight at K = 1 measured no faster on 57-diagonal bands and
+            // raised the rows kernel to 89 VGPRs.)
+            constexpr int DU = 4;
             for (int seg = 0; seg < nseg; seg++) {
                 const int nb = sh.ob + 32 * seg, ne = PM ? min(sh.oe, nb + 32) : sh.oe;
                 uint32_t w[P];
 #pragma unroll
                 for (int p = 0; p < P; p++) w[p] = seg ? phi[p] : plo[p];
                 int n = nb;
-                if constexpr (DU > 4) {
-                    for (; n + DU <= ne; n += DU) {
-                        int o[DU];
-#pragma unroll
-                        for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * K;
-                        float v[DU][P];
-#pragma unroll
-                        for (int u = 0; u < DU; u++)
-#pragma unroll
-                            for (int p = 0; p < P; p++) v[u][p] = s_val[li[p] + o[u]];
-#pragma unroll
-                        for (int u = 0; u < DU; u++)
-#pragma unroll
-                            for (int p = 0; p < P; p++)
-                                accum(acc[p], PM ? keep_if(v[u][p], w[p], n - nb + u) : v[u][p]);
-                    }
-                }
-                for (; n + 4 <= ne; n += 4) {
-                    constexpr int DU = 4;
+                for (; n + DU <= ne; n += DU) {
                     int o[DU];
 #pragma unroll
                     for (int u = 0; u < DU; u++) o[u] = cload(A.offs, n + u) * K;
@@ -752,19 +734,18 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
                     }
                 }
                 float xv[U];
-                if constexpr (sizeof(XT) == 4) {
+                if constexpr (sizeof(XT) == 4 && !FAST) {
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         xv[u] = 0.0f;
-                        if (b + u * kWG < ne) {
-                            const int col = vv[u] >> 1;
-                            if (FAST && (unsigned)(col - w0) < (unsigned)TRW) xv[u] = s_win[col - w0];
-                            else xv[u] = load_x<XT>(X, (size_t)col * ldx);
-                        }
+                        if (b + u * kWG < ne) xv[u] = load_x<XT>(X, (size_t)(vv[u] >> 1) * ldx);
                     }
                 } else {
-                    // 2-byte x: bits behind the branch, the window value and
-                    // the conversion after every gather is issued
+                    // the gathers' bits behind the branch; the window value
+                    // (FAST) and the conversion after every gather is issued.
+                    // A window read into the gather's own register would
+                    // have to wait for that gather (both write it), and a
+                    // lane's gathers would then run one after another.
                     uint32_t xb[U];
                     bool inw[U];
 #pragma unroll
@@ -1499,16 +1480,28 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                 if (j0 + U < re[i])
 #pragma unroll
                     for (int u = 0; u < U; u++) nx[u] = j0 + U + u < re[i] ? A.cols[j0 + U + u] : 1;
-                bool tl[U];
+                bool tl[U], nr[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     tl[u] = false;
-                    if (j0 + u < re[i]) {
+                    nr[u] = NEAR && (unsigned)((v[u] >> 1) - wl) < (unsigned)wn;
+                    if (j0 + u < re[i] && !nr[u]) {
                         const int c = v[u] >> 1;
-                        if (NEAR && (unsigned)(c - wl) < (unsigned)wn) ld_words(smw + (c - wl) * RW + c0w, xs[u].w);
-                        else if constexpr (MASK)
+                        if constexpr (MASK)
                             tl[u] = load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[u]);
                         else load_seg<XT, CPL>(X + (size_t)c * ldx + c0, xs[u]);
+                    }
+                }
+                // near entries read the staged rows after every gather is
+                // issued, into registers of their own (a read into the
+                // gather's register would wait for that gather)
+                if constexpr (NEAR) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        XSeg<XT, CPL> xn;
+                        ld_words(smw + (nr[u] ? (v[u] >> 1) - wl : 0) * RW + c0w, xn.w);
+#pragma unroll
+                        for (int w = 0; w < XSeg<XT, CPL>::NW; w++) xs[u].w[w] = nr[u] ? xn.w[w] : xs[u].w[w];
                     }
                 }
                 if constexpr (MASK) {
