@@ -135,7 +135,7 @@ int launch_step_t(const tcd::MatView &A, const float *zin, float *zout, float *r
     const int ntiles = (A.M + TRW - 1) / TRW;
     const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1 && A.sh0.pm < 0;
     const bool lng = A.L > 0;
-    const size_t lds = ((size_t)tcd::lsum_at(capv) + (lng ? 2 * TRW : 0) + (fast ? TRW : 0)) * 4;
+    const size_t lds = ((size_t)tcd::lsum_at(capv) + (lng ? 2 * TRW : 0)) * 4;
     if (lds + kWG * 4 > tcd::max_lds_bytes())
         return tc::fail(TC_ERR_UNSUPPORTED, "pagerank: tiles of %d rows with %d LDS values need %zu B", TRW, capv, lds);
 #define TC_STEP(OW, F, L) \

@@ -164,7 +164,7 @@ void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int n
     const int grid = units;
     if (kern == 0) {
         constexpr int TRW = kWG * kBatchRPT;
-        const size_t lds = ((size_t)lsum_at(kBatchCapv) + 3 * TRW) * 4;  // s_val | long sums (2*TRW floats reserved) | window
+        const size_t lds = ((size_t)lsum_at(kBatchCapv) + 2 * TRW) * 4;  // s_val | long sums (2*TRW floats reserved)
         batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     } else if (kern == 1 || kern == 3) {
         const int trw = wide_rows(K, kBatchRI, (int)sizeof(XT));

@@ -231,7 +231,7 @@ int launch_rows_t(const MatView &A, const void *X, int ldx, float *Y, int ldy, i
     if (ntiles == 0) return TC_OK;
     const int G = grid > 0 ? std::min(grid, ntiles) : ntiles;
     const bool fast = A.S == 1 && A.sh0.oe - A.sh0.ob == 1 && A.sh0.pm < 0;
-    const size_t lds = ((size_t)lsum_at(capv) + (A.L > 0 ? 2 * TRW : 0) + (fast ? TRW : 0)) * 4;
+    const size_t lds = ((size_t)lsum_at(capv) + (A.L > 0 ? 2 * TRW : 0)) * 4;
     if (lds > max_lds_bytes())
         return tc::fail(TC_ERR_UNSUPPORTED, "rows tiles of %d with %d LDS values need %zu B of LDS", TRW, capv, lds);
     const bool lng = A.L > 0;

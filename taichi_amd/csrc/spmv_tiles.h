@@ -607,7 +607,7 @@ __device__ __forceinline__ void run_tile(const MatView &A, const XT *__restrict_
 // two scalar loads (tiles are 64-row aligned). All loads of a phase are
 // issued before the first use. Same canonical order as run_tile. Column c of
 // X is X[c * ldx], row r of Y is Y[r * ldy] (1 and 1 for a plain vector).
-// LDS: s_val[capv] | s_lsum[TRW] (LONG; 2*TRW floats reserved) | s_win[TRW] (FAST).
+// LDS: s_val[capv] | s_lsum[TRW] (LONG; 2*TRW floats reserved).
 // PIPE (general path only): prefetch the next batch's column indices and sum
 // rows with four LDS reads in flight; the batched kernel, whose registers
 // serve every matrix kind in one launch, leaves it off.
@@ -630,7 +630,6 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
     constexpr int TRW = kWG * RPT;
     float *s_val = smem;
     acc_t *s_lsum = smem + lsum_at(capv);
-    float *s_win = smem + lsum_at(capv) + (LONG ? 2 * TRW : 0);
     const int tid = threadIdx.x, lane = tid & 63;
     const uint8_t *ro8 = (const uint8_t *)A.roff;
     const uint16_t *ro16 = (const uint16_t *)A.roff;
@@ -640,8 +639,8 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
         const int ebase = cload(A.gbase, u0 >> 6);
         const int E = cload(A.gbase, u0 + TRW >= A.M ? A.ngroups : (u0 + TRW) >> 6) - ebase;
 
-        // Issue order: x window (FAST), row offsets, the first batch of the
-        // column stream; the adds follow in the canonical order.
+        // Issue order: the diagonal's x (FAST), row offsets, the first batch
+        // of the column stream; the adds follow in the canonical order.
         constexpr int U = 4;
         int prow[RPT], pcol[RPT], lq[RPT], endv[RPT], gbv[RPT];
         bool pv[RPT];
@@ -655,8 +654,11 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
             pv[j] = u0 + prow[j] < A.M;
         }
         // FAST (one shape, one offset): x[w0 + i], i < TRW, holds this tile's
-        // diagonal terms; it is loaded once, kept in LDS (s_win) and also
-        // serves the remainder gathers that fall inside it.
+        // diagonal terms, one coalesced load per row. Remainder gathers that
+        // fall near the diagonal find those lines in L1; staging them in LDS
+        // for those gathers (a barrier and a select per gather) measured
+        // slower on R-MAT 2^22 (15.8 against 15.6 us FP32, 14.4 against 13.5
+        // BF16).
         const int w0 = FAST ? u0 + A.sh0.ys - A.sh0.xs + A.sh0.omin : 0;
         float xw[RPT];
         if constexpr (FAST) {
@@ -680,12 +682,10 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
             const ShapeDev &sh = A.sh0;
 #pragma unroll
             for (int j = 0; j < RPT; j++) {
-                s_win[prow[j]] = xw[j];
                 const int r = u0 + prow[j], c = w0 + prow[j];
                 if (pv[j] && r >= sh.xs && r < sh.xe && c >= sh.ys && c < sh.ys + (sh.xe - sh.xs))
                     accum(acc[j], xw[j]);
             }
-            __syncthreads();
         } else {
             diag_phase<1, XT, RPT>(A, X, ldx, u0, rows, prow, pcol, pv, acc, s_val, capv);
         }
@@ -734,32 +734,17 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
                     }
                 }
                 float xv[U];
-                if constexpr (sizeof(XT) == 4 && !FAST) {
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        xv[u] = 0.0f;
-                        if (b + u * kWG < ne) xv[u] = load_x<XT>(X, (size_t)(vv[u] >> 1) * ldx);
-                    }
-                } else {
-                    // the gathers' bits behind the branch; the window value
-                    // (FAST) and the conversion after every gather is issued.
-                    // A window read into the gather's own register would
-                    // have to wait for that gather (both write it), and a
-                    // lane's gathers would then run one after another.
+                {
+                    // 2-byte x: bits behind the branch, converted after every
+                    // gather is issued (a conversion inside it serialises them)
                     uint32_t xb[U];
-                    bool inw[U];
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        const int col = vv[u] >> 1;
-                        inw[u] = FAST && (unsigned)(col - w0) < (unsigned)TRW;
                         xb[u] = 0u;
-                        if (b + u * kWG < ne && !inw[u]) xb[u] = load_xbits<XT>(X, (size_t)col * ldx);
+                        if (b + u * kWG < ne) xb[u] = load_xbits<XT>(X, (size_t)(vv[u] >> 1) * ldx);
                     }
 #pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        xv[u] = xbits_f<XT>(xb[u]);
-                        if constexpr (FAST) xv[u] = inw[u] ? s_win[inw[u] ? (vv[u] >> 1) - w0 : 0] : xv[u];
-                    }
+                    for (int u = 0; u < U; u++) xv[u] = xbits_f<XT>(xb[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
@@ -779,7 +764,7 @@ __device__ __forceinline__ void rows_tile(const MatView &A, const XT *__restrict
                 }
                 acc[j] = a;
             }
-            // long_phase writes s_lsum only, apart from s_val and s_win
+            // long_phase writes s_lsum only, apart from s_val
             if (more || e0 + capv < E) __syncthreads();
         }
 
