@@ -206,3 +206,55 @@ def test_hand_made_layout_corners(dev):
     den = np.maximum(np.abs(y64), O.taichi_spmv(fa, np.abs(x64)))
     den[den == 0] = 1
     assert float(np.max(np.abs(y.cpu().numpy() - y64) / den)) <= 1e-6
+
+
+@pytest.mark.parametrize("n,w,extra", [(30000, 24, 0), (20000, 64, 0), (30000, 40, 5000)])
+def test_band_offset_order(n, w, extra, dev):
+    """A band's offsets stored in increasing order take the contiguous-offset
+    loop (compile-time offsets, one bit-select a term); the same format with
+    each shape's offsets stored in another order takes the general loop.
+    Both give the CPU sums exactly for integer x (k = 1..8, FP32 and BF16 x)
+    and stay within the FP64 bound for real x."""
+    rp, col = _band_plus_random(n, w, extra, 23)
+    fmt, _ = T.partition_shapes(n, n, rp, col)
+    assert fmt.dia_shapes >= 1
+    offs = np.asarray(fmt.neg_offsets).copy()
+    ptr = np.asarray(fmt.nDiasPtr)
+    assert all(np.array_equal(offs[ptr[s]:ptr[s + 1]], np.arange(offs[ptr[s]], offs[ptr[s]] + ptr[s + 1] - ptr[s]))
+               for s in range(fmt.dia_shapes))  # bands: consecutive increasing offsets
+    rng = np.random.default_rng(9)
+    perm = offs.copy()
+    for s in range(fmt.dia_shapes):
+        perm[ptr[s]:ptr[s + 1]] = rng.permutation(offs[ptr[s]:ptr[s + 1]])
+    fb = T.Format(fmt.M, fmt.N)
+    for name in T.Format.FIELDS + T.Format.ARRAYS:
+        setattr(fb, name, getattr(fmt, name))
+    fb.neg_offsets = perm.astype(np.int32)
+    A, B = T.Matrix(fmt, nnz=len(col)), T.Matrix(fb, nnz=len(col))
+    assert A.stats()["holes"] == B.stats()["holes"]
+    xi = T.gen_vector(n * 8, 1, 31).reshape(n, 8)
+    want = [O.csr_spmv_f32(rp, col, np.ascontiguousarray(xi[:, c])) for c in range(8)]
+    for dt in (torch.float32, torch.bfloat16):
+        Xd = torch.from_numpy(xi).to(dev).to(dt)  # integers 0..7 are exact in BF16
+        for k in (1, 2, 3, 4, 5, 8):
+            outs = []
+            for M_ in (A, B):
+                Y = torch.full((n, 8), float("nan"), device=dev)
+                if k == 1:
+                    y = torch.empty(n, device=dev)
+                    M_.spmv(Xd[:, 0].contiguous(), y)
+                    torch.cuda.synchronize()
+                    outs.append(y.cpu().numpy()[:, None])
+                else:
+                    M_.spmv(Xd[:, :k], Y[:, :k])
+                    torch.cuda.synchronize()
+                    outs.append(Y.cpu().numpy()[:, :k])
+            for c in range(k):
+                assert np.array_equal(outs[0][:, c], want[c]), (dt, k, c)
+                assert np.array_equal(outs[1][:, c], want[c]), (dt, k, c)
+    xr = np.random.default_rng(4).standard_normal(n).astype(np.float32)
+    for M_ in (A, B):
+        y = torch.empty(n, device=dev)
+        M_.spmv(torch.from_numpy(xr).to(dev), y)
+        torch.cuda.synchronize()
+        check_float(y.cpu().numpy(), rp, col, xr.astype(np.float64))

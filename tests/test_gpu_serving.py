@@ -95,3 +95,28 @@ def test_load_generator_open_and_closed(dev):
     p = srv.run_plain(items, ybufs, 8, count=100, seed=3)
     assert p["completed"] == 100
     srv.close()
+
+
+@pytest.mark.parametrize("k,ld", [(1, 1), (3, 4), (6, 8)])
+def test_load_generator_per_request_strides(k, ld, dev):
+    """ldy = 0: each request's y has the row stride of its own x (the serving
+    bench's per-request buffers, rows padded to the kernels' vector widths).
+    One item, so every request computes the same y: each output buffer
+    equals tc_spmv alone, and the padding columns stay untouched."""
+    n, rp, col, fmt = _mats(2)[1]  # a band
+    srv = T.Server(hbm_cap_bytes=1 << 30, max_batch=16, streams=2, max_inflight=32)
+    mid = srv.add_matrix(fmt)
+    X = torch.from_numpy(T.gen_vector(n * ld, 2, 77).reshape(n, ld)).to(dev)
+    ybufs = [torch.full((n, ld), float("nan"), device=dev) for _ in range(8)]
+    r = srv.run_load([(mid, X.data_ptr(), ld, T.F32, k)], ybufs, 0, count=64, rate=0.0, concurrency=8, seed=5)
+    assert r["completed"] == 64
+    p = srv.run_plain([(mid, X.data_ptr(), ld, T.F32, k)], ybufs, 0, count=16, seed=6)
+    assert p["completed"] == 16
+    torch.cuda.synchronize()
+    Ya = torch.full((n, ld), float("nan"), device=dev)
+    T.Matrix(fmt).spmv(X[:, :k], Ya[:, :k])
+    torch.cuda.synchronize()
+    for Y in ybufs:
+        assert torch.equal(Y[:, :k], Ya[:, :k])
+        assert torch.isnan(Y[:, k:]).all()
+    srv.close()
