@@ -292,6 +292,18 @@ __device__ __forceinline__ void block_units(int total, int *b, int *e)
     }
 }
 
+// f(integral_constant<JB>) for JB = 0, DU, 2 DU, ... below min(nd, 64):
+// a loop over groups of DU offsets unrolled at compile time
+template <int DU, int JB = 0, class F>
+__device__ __forceinline__ void offset_groups(int nd, F &&f)
+{
+    if constexpr (JB < 64) {
+        if (JB >= nd) return;
+        f(std::integral_constant<int, JB>{});
+        offset_groups<DU, JB + DU>(nd, f);
+    }
+}
+
 // acc += s[i * K] for i in [lo, hi), in order, with four LDS reads in flight
 // (a lane's serial sum of its row's staged entries)
 template <int K>
@@ -406,7 +418,40 @@ __device__ __forceinline__ void diag_one_shape_t(const MatView &A, const XT *__r
         int li[P];
 #pragma unroll
         for (int p = 0; p < P; p++) li[p] = (base[p] - w0) * K + pcol[p];
-        if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
+        const int nd = sh.oe - sh.ob;
+        if (__builtin_amdgcn_ballot_w64(!inner) == 0 && sh.seq && nd <= 64 && (w1 - w0 + 8) * K <= capv) {
+            // contiguous offsets omin + j (a band): offset j of a pair reads
+            // the window at its base + j K, a compile-time offset (the LDS
+            // instruction's immediate), with no scalar load of offsets in
+            // the loop; absent terms are masked (keep_if). Groups of DU
+            // offsets, reads of a group in flight together; a group past
+            // the last offset reads rows the staged window holds room for
+            // (8 rows) and masks them (their bits are clear).
+            constexpr int DU = P <= 2 ? 8 : 4;
+            const uint32_t klo = nd >= 32 ? ~0u : (1u << nd) - 1u;
+            const uint32_t khi = nd >= 64 ? ~0u : nd > 32 ? (1u << (nd - 32)) - 1u : 0u;
+            uint32_t mlo[P], mhi[P];
+            int lb[P];
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                mlo[p] = (PM ? plo[p] : ~0u) & klo;
+                mhi[p] = (PM ? phi[p] : ~0u) & khi;
+                lb[p] = li[p] + sh.omin * K;
+            }
+            offset_groups<DU>(nd, [&](auto jbc) {
+                constexpr int jb = decltype(jbc)::value;
+                float v[DU][P];
+#pragma unroll
+                for (int u = 0; u < DU; u++)
+#pragma unroll
+                    for (int p = 0; p < P; p++) v[u][p] = s_val[lb[p] + (jb + u) * K];
+#pragma unroll
+                for (int u = 0; u < DU; u++)
+#pragma unroll
+                    for (int p = 0; p < P; p++)
+                        accum(acc[p], keep_if(v[u][p], (jb + u) < 32 ? mlo[p] : mhi[p], (jb + u) & 31));
+            });
+        } else if (__builtin_amdgcn_ballot_w64(!inner) == 0) {
             // DU offsets per step: their scalar loads, then all DU * P LDS
             // reads, then the adds (each pair in offset order). (16 offsets
             // in flight at K = 1 measured no faster on 57-diagonal bands and
@@ -1119,18 +1164,6 @@ __device__ __forceinline__ void stage_seg_rows(const XT *__restrict__ X, int ldx
             const int v = min(b + u * kWG + tid, nw - 1);
             st_words(smw + (v / LPR) * RW + (v % LPR) * SW, tv[u].w);
         }
-    }
-}
-
-// f(integral_constant<JB>) for JB = 0, DU, 2 DU, ... below min(nd, 64):
-// a loop over groups of DU offsets unrolled at compile time
-template <int DU, int JB = 0, class F>
-__device__ __forceinline__ void offset_groups(int nd, F &&f)
-{
-    if constexpr (JB < 64) {
-        if (JB >= nd) return;
-        f(std::integral_constant<int, JB>{});
-        offset_groups<DU, JB + DU>(nd, f);
     }
 }
 
