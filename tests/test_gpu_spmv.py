@@ -266,6 +266,41 @@ def test_half_storage(dt, dev):
     check_float(y, rp, col, xq.astype(np.float64))
 
 
+@pytest.mark.parametrize("kind", ["rmat", "stencil", "band"])
+@pytest.mark.parametrize("dt", [torch.bfloat16, torch.float16])
+def test_half_x_at_odd_offset(kind, dt, dev):
+    """2-byte x gathers load the aligned 4-byte word that holds the element
+    and pick its half: x starting at an odd element (the word's upper half
+    for even i) and at an even one give the same y, bit for bit, for every
+    k = 1 tile size, and integer x matches the CPU sum exactly."""
+    if kind == "rmat":
+        M, N, rp, col = rmat(15)
+    elif kind == "stencil":  # 5-point stencil + 2 % uniform: all in the remainder
+        M = N = 300 * 300
+        rp, col = T.gen_stencil5(300, 300, 0.02, 2)
+    else:
+        M = N = 40000
+        rp, col = T.gen_banded(M, 40, 0.7, 5)
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    A = T.Matrix(fmt)
+    xi = np.random.default_rng(N).integers(0, 8, N).astype(np.float32)
+    want = O.csr_spmv_f32(rp, col, xi)
+    buf = torch.full((N + 3,), 7.0, device=dev, dtype=dt)  # neighbours of x hold 7
+    code = T.BF16 if dt == torch.bfloat16 else T.F16
+    for rows, fl in ((256, 2), (512, 2), (1024, 2), (2048, 2), (512, 0), (1024, 0)):
+        A.set_config(1, code, rows, 4096, 0, fl)
+        ys = []
+        for off in (1, 2):
+            buf[off:off + N] = torch.from_numpy(xi).to(dev).to(dt)
+            y = torch.full((M,), float("nan"), device=dev)
+            A.spmv(buf[off:off + N], y)
+            torch.cuda.synchronize()
+            ys.append(y.cpu().numpy())
+            buf.fill_(7.0)
+        assert np.array_equal(ys[0], want), (rows, fl)
+        assert np.array_equal(ys[1], want), (rows, fl)
+
+
 def test_batch_matches_alone(dev):
     mats = []
     for i, (scale, ef) in enumerate([(12, 4), (14, 16), (16, 8), (13, 32)]):
