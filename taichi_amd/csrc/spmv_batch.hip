@@ -16,7 +16,16 @@ namespace tcd {
 // change nothing in the result: the per-(row, column) order is independent
 // of K.
 constexpr int kBatchP = 2;       // K > 1: run_tile with P = 2 (512/K rows a tile)
-constexpr int kBatchRPT = 2;     // K = 1: rows_tile with 512 rows a tile
+// K = 1: rows_tile with 1024 rows a tile when the group holds at least
+// kBatchBigRows rows (four thousand tiles: two full rounds of the GPU), else
+// 512. On 256 uneven R-MAT matrices (16 M rows) 1024-row tiles take 70 us a
+// batch, against 101 us with 512 rows and 85 us with 2048; a 2048- or
+// 8192-value stage instead of 4096: 69 and 106 us. A lone request keeps
+// 512-row tiles, twice the workgroups: at 1024 the served mix's p99 at 20k
+// req/s went from 102-112 to 132-134 us.
+constexpr int kBatchRPT = 2;
+constexpr int kBatchRPTBig = 4;
+constexpr long long kBatchBigRows = 4ll << 20;
 constexpr int kBatchCapv = 4096; // staged values per pass
 constexpr int kBatchLdsDesc = 512;
 
@@ -73,7 +82,7 @@ __device__ __forceinline__ void batch_loop(const WorkDesc *__restrict__ D, const
     }
 }
 
-template <typename XT>
+template <typename XT, int RPT>
 __global__ __launch_bounds__(kWG) void batch_rows_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
                                                          int nd, int total, int capv)
 {
@@ -83,15 +92,15 @@ __global__ __launch_bounds__(kWG) void batch_rows_kernel(const WorkDesc *__restr
         const XT *X = (const XT *)w.X;
         const bool lng = w.A.L > 0;
         if (w.A.ow == 1) {
-            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 1, true, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                            : rows_tile<XT, kBatchRPT, 1, true, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
-            else lng ? rows_tile<XT, kBatchRPT, 1, false, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                     : rows_tile<XT, kBatchRPT, 1, false, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            if (w.fast) lng ? rows_tile<XT, RPT, 1, true, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, RPT, 1, true, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, RPT, 1, false, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, RPT, 1, false, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
         } else {
-            if (w.fast) lng ? rows_tile<XT, kBatchRPT, 2, true, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                            : rows_tile<XT, kBatchRPT, 2, true, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
-            else lng ? rows_tile<XT, kBatchRPT, 2, false, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
-                     : rows_tile<XT, kBatchRPT, 2, false, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            if (w.fast) lng ? rows_tile<XT, RPT, 2, true, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                            : rows_tile<XT, RPT, 2, true, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
+            else lng ? rows_tile<XT, RPT, 2, false, true, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv)
+                     : rows_tile<XT, RPT, 2, false, false, false>(w.A, X, w.ldx, w.Y, w.ldy, t, smem, capv);
         }
     });
 }
@@ -149,23 +158,26 @@ size_t batch_desc_bytes(int max_requests)
 constexpr int kBatchClasses = 4 * 4 * 3;
 inline int batch_class(int kern, int ki, int dt) { return (kern * 4 + ki) * 3 + dt; }
 
-int batch_tile_rows(int kern, int K, int esize)
+// big: a K = 1 group of at least kBatchBigRows rows
+int batch_tile_rows(int kern, int K, int esize, bool big)
 {
-    if (kern == 0) return kWG * kBatchRPT;
+    if (kern == 0) return kWG * (big ? kBatchRPTBig : kBatchRPT);
     if (kern == 1 || kern == 3) return wide_rows(K, kBatchRI, esize);
     return (kWG * kBatchP) / K;
 }
 
 template <typename XT>
-void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int nd, int units, hipStream_t s)
+void launch_batch_class(int kern, int K, const WorkDesc *D, const int *UB, int nd, int units, bool big,
+                        hipStream_t s)
 {
     // one workgroup per tile: the dispatcher balances chunks of any size (a
     // grid of 8 workgroups per CU walking runs of tiles measured slower)
     const int grid = units;
     if (kern == 0) {
-        constexpr int TRW = kWG * kBatchRPT;
-        const size_t lds = ((size_t)lsum_at(kBatchCapv) + 2 * TRW) * 4;  // s_val | long sums (2*TRW floats reserved)
-        batch_rows_kernel<XT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+        const int trw = kWG * (big ? kBatchRPTBig : kBatchRPT);
+        const size_t lds = ((size_t)lsum_at(kBatchCapv) + 2 * trw) * 4;  // s_val | long sums (2*TRW floats reserved)
+        if (big) batch_rows_kernel<XT, kBatchRPTBig><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
+        else batch_rows_kernel<XT, kBatchRPT><<<grid, kWG, lds, s>>>(D, UB, nd, units, kBatchCapv);
     } else if (kern == 1 || kern == 3) {
         const int trw = wide_rows(K, kBatchRI, (int)sizeof(XT));
         const size_t lds = wide_lds_bytes(K, trw, false, true);  // batched tiles run without the near window
@@ -235,6 +247,7 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
             }
     };
     int cls_count[kBatchClasses] = {0};
+    long long cls_rows[kBatchClasses] = {0};
     int nchunks = 0;
     for (int i = 0; i < n; i++) {
         const tc_request &r = reqs[i];
@@ -243,7 +256,9 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
         if (r.k > kMaxBatchK) return tc::fail(TC_ERR_UNSUPPORTED, "batch: k=%d > %d", r.k, kMaxBatchK);
         const int dt = r.dtype == TC_F32 ? 0 : r.dtype == TC_BF16 ? 1 : 2;
         chunks(r, [&](int ki, int K, int c0, int kv) {
-            cls_count[chunk_class(r, ki, K, c0, dt, kv)]++;
+            const int cl = chunk_class(r, ki, K, c0, dt, kv);
+            cls_count[cl]++;
+            cls_rows[cl] += r.A->v.M;
             nchunks++;
         });
     }
@@ -278,7 +293,7 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
             w.ldy = r.ldy;
             w.fast = fast ? 1 : 0;
             w.kv = kv;
-            const int trw = batch_tile_rows(cl / 12, K, elem_size(r.dtype));
+            const int trw = batch_tile_rows(cl / 12, K, elem_size(r.dtype), cls_rows[cl] >= kBatchBigRows);
             w.unit_begin = (int)units[cl];
             UB[fill[cl]] = w.unit_begin;
             units[cl] += (v.M + trw - 1) / trw;
@@ -293,9 +308,10 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
     for (int c = 0; c < kBatchClasses; c++) {
         if (units[c] == 0) continue;
         const int kern = c / 12, K = kChunkK[(c / 3) % 4], dt = c % 3, nd = cls_count[c], b = cls_begin[c];
-        if (dt == 0) launch_batch_class<float>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
-        else if (dt == 1) launch_batch_class<bf16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
-        else launch_batch_class<f16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], s);
+        const bool big = cls_rows[c] >= kBatchBigRows;
+        if (dt == 0) launch_batch_class<float>(kern, K, dD + b, dUB + b, nd, (int)units[c], big, s);
+        else if (dt == 1) launch_batch_class<bf16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], big, s);
+        else launch_batch_class<f16_t>(kern, K, dD + b, dUB + b, nd, (int)units[c], big, s);
         HIPCHECK(hipGetLastError());
     }
     return TC_OK;
