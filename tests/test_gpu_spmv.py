@@ -325,6 +325,31 @@ def test_batch_matches_alone(dev):
         assert torch.equal(Yb, Ya)
 
 
+@pytest.mark.parametrize("count", [3, 5])
+def test_batch_one_column_tile_sizes(count, dev):
+    """One-column groups of 4 M rows or more run 1024-row tiles, smaller ones
+    512-row tiles (3 and 5 matrices of about 1 M rows fall either side):
+    every y equals the matrix run alone, bit for bit, and integer x equals the
+    CPU sum."""
+    items, want = [], []
+    for i in range(count):
+        n = (1 << 20) - 77 * i
+        rp, col = T.gen_rmat(n, 20, 8 * n, 0.499, 0.001, 0.001, 300 + i)
+        A = T.Matrix(host_format(n, n, rp, col))
+        x = T.gen_vector(n, 1, 400 + i)  # integers 0..7
+        X = torch.from_numpy(x).to(dev)
+        Ya = torch.full((n,), float("nan"), device=dev)
+        A.spmv(X, Ya)
+        items.append((A, X, torch.full((n,), float("nan"), device=dev)))
+        want.append((Ya, O.csr_spmv_f32(rp, col, x)))
+    assert (sum(A.stats()["M"] for A, _, _ in items) >= 4 << 20) == (count == 5)
+    T.spmv_batch(items)
+    torch.cuda.synchronize()
+    for (A, X, Yb), (Ya, yc) in zip(items, want):
+        assert torch.equal(Yb, Ya)
+        assert np.array_equal(Yb.cpu().numpy(), yc)
+
+
 @pytest.mark.parametrize("ks", [(3, 7, 13, 20), (64, 1, 6, 11)])
 def test_batch_column_chunks(dev, ks):
     """Requests wider than a chunk are cut into 8/4/2/1-column pieces, x types
