@@ -35,6 +35,9 @@ int cu_count()
 // diagonal terms together average more than 24 a row (wide bands), use
 // 256-row tiles at k = 1: a tile's entries pass through the stage in about
 // two rounds, and the tiles are short enough to spread the band's ends.
+// Matrices below 2^18 rows also use 256-row tiles at k = 1: twice the
+// workgroups for a launch that cannot fill the GPU (n = 16,384, 5 diagonals
+// and 16,384 uniform entries: 3.5 against 4.1 us).
 // Two rows per lane group (a config choice) measured up to 17 % slower than
 // one on bands of 4 to 57 diagonals at k = 4..8 (57 diagonals of 923k rows,
 // BF16 k = 6: 65 against 54 us), 2-3 % faster only for FP32 k = 6 on two.
@@ -43,7 +46,8 @@ tc_config default_config(int K, int esize, long long remainder, long long diag_t
     tc_config c{};
     const bool band = M > 0 && remainder + diag_terms > 24ll * M;
     const bool heavy = M > 0 && (remainder > 8ll * M || band);
-    c.rows_per_tile = K == 1      ? (heavy ? 256 : 512)
+    const bool small = M < (1 << 18);
+    c.rows_per_tile = K == 1      ? (heavy || small ? 256 : 512)
                       : wide_k(K) ? wide_rows(K, 1, esize)
                                   : (kWG * 2) / K;
     c.lds_values = 4096;
