@@ -128,11 +128,14 @@ void dispatcher(tc_server *S)
         // 64-matrix serving mix both give the same throughput (85.8k against
         // 85.4k req/s), and the coalesced form launches a few kernels a
         // batch instead of one a request.
+        // A batch of one request (most of them at light load) launches it
+        // as tc_spmv: no descriptor copy ahead of its kernel.
         int rc = TC_OK;
         int nsmall = 0;
+        const bool alone = S->cfg.launch == 2 || take == 1;
         for (int i = 0; i < take && rc == TC_OK; i++) {
             const tc_request &q = reqs[i];
-            if (S->cfg.launch == 2) rc = tc_spmv(q.A, q.X, q.ldx, q.dtype, q.Y, q.ldy, q.k, sl.stream);
+            if (alone) rc = tc_spmv(q.A, q.X, q.ldx, q.dtype, q.Y, q.ldy, q.k, sl.stream);
             else reqs[nsmall++] = q;
         }
         if (rc == TC_OK && nsmall > 0) rc = tc::batch_enqueue(reqs.data(), nsmall, sl.host_desc, sl.dev_desc, sl.stream, sl.desc_cap);
