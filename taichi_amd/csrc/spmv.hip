@@ -316,7 +316,8 @@ try {
         const int W = masked_width(K);
         if (W) {
             const tc_config cw = get_cfg(A, W, dtype);
-            if (!(cw.flags & kNoMask) && wide_applies(W, xc, ldx, yc, ldy, (int)es, cw)) {
+            // (a masked pass reads X's last row through the row above)
+            if (!(cw.flags & kNoMask) && A->v.N >= 2 && wide_applies(W, xc, ldx, yc, ldy, (int)es, cw)) {
                 rc = launch_wide(A->v, xc, ldx, yc, ldy, dtype, W, cw, s, K);
                 if (rc != TC_OK) return rc;
                 c0 += K;

@@ -236,7 +236,7 @@ int batch_enqueue(const tc_request *reqs, int n, void *host_desc, void *dev_desc
         // 3, 5, 6 or 7 columns run as one masked pass of width 4 or 8 (a
         // lane owns all 8 columns of a row, for every x type)
         const int W = rest > 0 ? masked_width(rest) : 0;
-        if (W && chunk_aligned(r, W, c0) && !(get_cfg(r.A, W, r.dtype).flags & kNoMask)) {
+        if (W && r.A->v.N >= 2 && chunk_aligned(r, W, c0) && !(get_cfg(r.A, W, r.dtype).flags & kNoMask)) {
             f(W == 2 ? 1 : W == 4 ? 2 : 3, W, c0, rest);
             return;
         }

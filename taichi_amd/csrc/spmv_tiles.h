@@ -907,7 +907,6 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
 // 32 bytes of zeros, the in-bounds stand-in for a vector load that would run
 // past the end of X (global, not constant, memory: a select between it and an
 // x address stays a global load)
-static __device__ __align__(16) uint32_t kXPad[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 
 // A raw x row segment of Q elements as loaded: Q words for 4-byte x, Q/2
 // words (two elements each) for 2-byte x. Gathers fill segments and the adds
@@ -969,18 +968,21 @@ __device__ __forceinline__ void load_seg(const XT *__restrict__ p, XSeg<XT, Q> &
 // Segment of a masked chunk. The whole vector is loaded: it lies inside the
 // row's ldx span (ldx is a multiple of Q and the lane's first column is below
 // kv <= ldx), except in the last row of X, which may end at column kv --
-// there (the returned tail: last, nv < Q) the vector comes from kXPad, and
+// there (the returned tail: last, nv < Q) the vector is read from the row
+// above, the same columns one ldx earlier (masked chunks need N >= 2), and
 // seg_m_tail later puts the first nv elements in, read one by one (the
 // others stay 0). Elsewhere the elements at or past nv are whatever X holds
 // there: they feed only columns that are never stored. The vector load is
-// unconditional (an address select). Callers issue a whole batch of loads
+// unconditional (an address select between two offsets of p: no pad array
+// whose address the loop would rebuild). Callers issue a whole batch of loads
 // before the rare tail fix-ups: a fix-up writes the segment's registers, so
 // it waits for the loads, and one inside the batch would serialise them.
 template <typename XT, int Q>
-__device__ __forceinline__ bool load_seg_m(const XT *__restrict__ p, int nv, bool last, XSeg<XT, Q> &s)
+__device__ __forceinline__ bool load_seg_m(const XT *__restrict__ X, int row, int ldx, int c0, int nv, int N,
+                                           XSeg<XT, Q> &s)
 {
-    const bool tail = last && nv < Q;
-    ld_words(tail ? (const void *)kXPad : (const void *)p, s.w);
+    const bool tail = row == N - 1 && nv < Q;
+    ld_words(X + (size_t)(row - (int)tail) * ldx + c0, s.w);
     return tail;
 }
 template <typename XT, int Q>
@@ -1048,7 +1050,8 @@ __host__ __device__ constexpr int wide_cpl(int K, int esize)
 // columns but stores only those below kv (load_seg_m keeps every load in
 // bounds). A masked chunk runs k = 3, 5, 6, 7 as one pass of width 4 or 8
 // instead of 2 + 1, 4 + 1, 4 + 2, 4 + 2 + 1 passes. Needs kv > K/2 (so a
-// lane's vector stays inside its X row, or X's last row is handled apart).
+// lane's vector stays inside its X row, or X's last row is handled apart)
+// and N >= 2 (X's last row borrows the row above for its vector loads).
 // LDS: a shape's x window (a run of kWideZero floats of +0.0, then up to
 // TRW + kWideHalo rows of K columns; for K = 8 two planes of 4 columns,
 // wide_plane floats apart), reused for the rows around the tile that a near
@@ -1095,7 +1098,7 @@ __device__ __forceinline__ void stage_x_rows(const XT *__restrict__ X, int ldx, 
             const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
             tl[u] = false;
             if constexpr (MASK)
-                tl[u] = load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
+                tl[u] = load_seg_m<XT, CPL>(X, w0 + v / LPR, ldx, (v % LPR) * CPL, min(max(kv - (v % LPR) * CPL, 0), CPL), N, tv[u]);
             else load_seg<XT, CPL>(xp, tv[u]);
         }
         if constexpr (MASK) {
@@ -1147,7 +1150,7 @@ __device__ __forceinline__ void stage_seg_rows(const XT *__restrict__ X, int ldx
             const XT *xp = X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL;
             tl[u] = false;
             if constexpr (MASK)
-                tl[u] = load_seg_m<XT, CPL>(xp, min(max(kv - (v % LPR) * CPL, 0), CPL), w0 + v / LPR == N - 1, tv[u]);
+                tl[u] = load_seg_m<XT, CPL>(X, w0 + v / LPR, ldx, (v % LPR) * CPL, min(max(kv - (v % LPR) * CPL, 0), CPL), N, tv[u]);
             else load_seg<XT, CPL>(xp, tv[u]);
         }
         if constexpr (MASK) {
@@ -1441,7 +1444,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                         tl[i] = false;
                         if (in[i]) {
                             if constexpr (MASK)
-                                tl[i] = load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[i]);
+                                tl[i] = load_seg_m<XT, CPL>(X, c, ldx, c0, nv, A.N, xs[i]);
                             else load_seg<XT, CPL>(X + (size_t)c * ldx + c0, xs[i]);
                         }
                     }
@@ -1506,7 +1509,7 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     if (j0 + u < re[i] && !nr[u]) {
                         const int c = v[u] >> 1;
                         if constexpr (MASK)
-                            tl[u] = load_seg_m<XT, CPL>(X + (size_t)c * ldx + c0, nv, c == A.N - 1, xs[u]);
+                            tl[u] = load_seg_m<XT, CPL>(X, c, ldx, c0, nv, A.N, xs[u]);
                         else load_seg<XT, CPL>(X + (size_t)c * ldx + c0, xs[u]);
                     }
                 }
