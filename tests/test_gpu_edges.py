@@ -46,6 +46,31 @@ def test_empty_and_nearly_empty(M, N, nnz, dev):
             assert np.array_equal(got, want), (k, dt)
 
 
+@pytest.mark.parametrize("k", [3, 5, 7])
+def test_single_row_x_with_masked_widths(k, dev):
+    """N = 1 (X one row): 3, 5 or 7 columns, which a masked pass would take,
+    run as unmasked chunks, alone and batched; columns past k stay
+    untouched, and NaN past column k of X is never read into a result."""
+    M, N = 3000, 1
+    rows = [np.array([0], np.int32) if r % 3 else np.zeros(0, np.int32) for r in range(M)]
+    rp, col = _csr(M, rows)
+    fmt, _ = T.partition_shapes(M, N, rp, col)
+    A = T.Matrix(fmt)
+    want = (np.arange(M) % 3 != 0).astype(np.float32)[:, None] * np.arange(1, k + 1, dtype=np.float32)[None, :]
+    for dt in (torch.float32, torch.bfloat16):
+        buf = torch.full((1, 8), float("nan"), device=dev, dtype=dt)
+        buf[0, :k] = torch.arange(1, k + 1, device=dev, dtype=torch.float32).to(dt)
+        for batched in (False, True):
+            Y = torch.full((M, 8), float("nan"), device=dev)
+            if batched:
+                T.spmv_batch([(A, buf[:, :k], Y[:, :k])])
+            else:
+                A.spmv(buf[:, :k], Y[:, :k])
+            torch.cuda.synchronize()
+            assert np.array_equal(Y[:, :k].cpu().numpy(), want), (dt, batched)
+            assert torch.isnan(Y[:, k:]).all(), (dt, batched)
+
+
 @pytest.mark.parametrize("shape", ["tall", "wide"])
 def test_non_square(shape, dev):
     M, N = (30000, 1000) if shape == "tall" else (1000, 300000)
