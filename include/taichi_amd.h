@@ -279,7 +279,8 @@ typedef struct tc_server_config {
     int max_inflight;         /* requests on the GPU at once */
     int launch;               /* how a batch is launched on its slot's stream:
                                  0 and 1: as one batched launch group (the
-                                 requests coalesced, tc_spmv_batch's kernels);
+                                 requests coalesced, tc_spmv_batch's kernels),
+                                 a batch of one request as tc_spmv;
                                  2: one by one (tc_spmv). Same results either
                                  way. */
 } tc_server_config;
