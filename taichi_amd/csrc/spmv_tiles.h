@@ -1102,13 +1102,16 @@ __device__ __forceinline__ void stage_x_rows(const XT *__restrict__ X, int ldx, 
             else load_seg<XT, CPL>(xp, tv[u]);
         }
         if constexpr (MASK) {
+            // one wave-uniform test skips the fix-ups when no lane has a tail
+            if (__builtin_amdgcn_ballot_w64(tl[0] || tl[1] || tl[2] || tl[3]) != 0) {
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (tl[u]) {
-                    const int v = min(b + u * kWG + tid, nw - 1);
-                    seg_m_tail<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL,
-                                        min(max(kv - (v % LPR) * CPL, 0), CPL), tv[u]);
-                }
+                for (int u = 0; u < 4; u++)
+                    if (tl[u]) {
+                        const int v = min(b + u * kWG + tid, nw - 1);
+                        seg_m_tail<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL,
+                                            min(max(kv - (v % LPR) * CPL, 0), CPL), tv[u]);
+                    }
+            }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -1154,13 +1157,16 @@ __device__ __forceinline__ void stage_seg_rows(const XT *__restrict__ X, int ldx
             else load_seg<XT, CPL>(xp, tv[u]);
         }
         if constexpr (MASK) {
+            // one wave-uniform test skips the fix-ups when no lane has a tail
+            if (__builtin_amdgcn_ballot_w64(tl[0] || tl[1] || tl[2] || tl[3]) != 0) {
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (tl[u]) {
-                    const int v = min(b + u * kWG + tid, nw - 1);
-                    seg_m_tail<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL,
-                                        min(max(kv - (v % LPR) * CPL, 0), CPL), tv[u]);
-                }
+                for (int u = 0; u < 4; u++)
+                    if (tl[u]) {
+                        const int v = min(b + u * kWG + tid, nw - 1);
+                        seg_m_tail<XT, CPL>(X + (size_t)(w0 + v / LPR) * ldx + (v % LPR) * CPL,
+                                            min(max(kv - (v % LPR) * CPL, 0), CPL), tv[u]);
+                    }
+            }
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -1526,9 +1532,14 @@ __device__ __forceinline__ void wide_tile(const MatView &A, const XT *__restrict
                     }
                 }
                 if constexpr (MASK) {
+                    bool any = false;
 #pragma unroll
-                    for (int u = 0; u < U; u++)
-                        if (tl[u]) seg_m_tail<XT, CPL>(X + (size_t)(A.N - 1) * ldx + c0, nv, xs[u]);
+                    for (int u = 0; u < U; u++) any = any || tl[u];
+                    if (__builtin_amdgcn_ballot_w64(any) != 0) {
+#pragma unroll
+                        for (int u = 0; u < U; u++)
+                            if (tl[u]) seg_m_tail<XT, CPL>(X + (size_t)(A.N - 1) * ldx + c0, nv, xs[u]);
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++)
