@@ -51,7 +51,10 @@ tc_config default_config(int K, int esize, long long remainder, long long diag_t
                       : wide_k(K) ? wide_rows(K, 1, esize)
                                   : (kWG * 2) / K;
     c.lds_values = 4096;
-    c.grid = 0;
+    // one workgroup per tile, except FP32 x at k = 64: a persistent grid of 8
+    // workgroups per CU (2048 x 2048 stencil: 0.563 against 0.593 ms; with
+    // BF16 x it is slower, 0.503 against 0.435 ms)
+    c.grid = K == 64 && esize == 4 ? 8 * cu_count() : 0;
     // k = 1: one row per lane (quad rows measured no faster on R-MAT, and
     // slower on the 2048^2 stencil: 35.7 against 28.5 us FP32, 34.9 against
     // 31.0 us BF16); the autotuner tries both
