@@ -53,6 +53,31 @@ def test_served_results_equal_alone(launch, dev):
     srv.close()
 
 
+def test_lone_requests_equal_alone(dev):
+    """Requests submitted one at a time, each waited for, make batches of one,
+    which launch as tc_spmv: every k in 1..8 and x type equals the matrix run
+    alone, and the padding columns of Y stay untouched."""
+    mats = _mats(2)
+    srv = T.Server(hbm_cap_bytes=1 << 30, max_batch=16, streams=2, max_inflight=16)
+    ids = [srv.add_matrix(fmt) for (_, _, _, fmt) in mats]
+    for i, (n, rp, col, fmt) in enumerate(mats):
+        A = T.Matrix(fmt)
+        for k in range(1, 9):
+            for dt in (torch.float32, torch.bfloat16):
+                X = torch.from_numpy(T.gen_vector(n * 8, 2, 50 * k + i).reshape(n, 8)).to(dev).to(dt)
+                Y = torch.full((n, 8), float("nan"), device=dev)
+                srv.wait(srv.submit(ids[i], X[:, :k], Y[:, :k]))
+                torch.cuda.synchronize()
+                Ya = torch.full((n, 8), float("nan"), device=dev)
+                A.spmv(X[:, :k], Ya[:, :k])
+                torch.cuda.synchronize()
+                assert torch.equal(Y[:, :k], Ya[:, :k]), (i, k, dt)
+                assert torch.isnan(Y[:, k:]).all(), (i, k, dt)
+    st = srv.stats()
+    assert st["completed"] == 2 * 8 * 2 and st["batches"] == st["completed"]
+    srv.close()
+
+
 def test_admission_by_footprint(dev):
     mats = _mats(4)
     fps = [T.footprint(fmt) for (_, _, _, fmt) in mats]
