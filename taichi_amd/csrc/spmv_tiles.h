@@ -904,10 +904,6 @@ __device__ __forceinline__ void load_q(const XT *__restrict__ p, float (&o)[Q])
     }
 }
 
-// 32 bytes of zeros, the in-bounds stand-in for a vector load that would run
-// past the end of X (global, not constant, memory: a select between it and an
-// x address stays a global load)
-
 // A raw x row segment of Q elements as loaded: Q words for 4-byte x, Q/2
 // words (two elements each) for 2-byte x. Gathers fill segments and the adds
 // convert them (seg_at), so no conversion sits between one gather and the
