@@ -34,6 +34,14 @@ def test_c_abi_exports_every_declaration():
     assert not missing, missing
 
 
+def test_stream_calibration_library_loads():
+    """The benchmark's streaming-ceiling kernels build into a library of their
+    own, outside the SpMV interface."""
+    lib = C.CDLL(os.path.join(ROOT, "taichi_amd", "libtc_stream.so"))
+    assert hasattr(lib, "tcs_copy") and hasattr(lib, "tcs_read")
+    assert not hasattr(lib, "tc_spmv")
+
+
 def test_cpp_surface_exports_every_declaration():
     """TaiChi.h keeps the reference's C++ signatures (TaiChi.h:79-107); each
     declared function is defined in the library with C++ linkage."""
