@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restr
 constexpr int kBatchRI = 1;  // one row per lane group: 42-56 VGPRs, full occupancy
 
 template <int K, typename XT, bool MASK>
-__global__ __launch_bounds__(kWG) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
+__global__ __launch_bounds__(kWG, K == 8 ? 7 : 8) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
                                                          int nd, int total)
 {
     extern __shared__ float smem[];
