@@ -6,8 +6,18 @@
 
 namespace tcd {
 
+// Waves a SIMD the compiler must fit the one-row-per-group tiles into
+// (scalar registers beyond 96 alone cost these kernels a wave): 8, or 7 at
+// 8 columns. The near-window tiles at 8 columns (77-85 vector registers) and
+// the wider tiles (RI = 2) keep the compiler's own choice: a bound there
+// spills to scratch memory.
+constexpr int wide_waves(int K, int RI, bool NEAR)
+{
+    return RI != 1 || (K == 8 && NEAR) ? 1 : K == 8 ? 7 : 8;
+}
+
 template <int K, typename XT, int RI, bool LONG, bool MASK, bool NEAR>
-__global__ __launch_bounds__(kWG) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
+__global__ __launch_bounds__(kWG, wide_waves(K, RI, NEAR)) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
                                                         float *__restrict__ Y, int ldy, int ntiles, int kv)
 {
     extern __shared__ float smem[];
