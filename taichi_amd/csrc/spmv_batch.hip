@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kWG) void batch_tile_kernel(const WorkDesc *__restr
 }
 
 // K > 1 chunks whose X and Y rows split into aligned vectors: wide-rows tiles
-constexpr int kBatchRI = 1;  // one row per lane group: 42-56 VGPRs, full occupancy
+constexpr int kBatchRI = 1;  // one row per lane group: 39-75 VGPRs
 
 // Waves a SIMD the compiler must fit the tiles into (scalar registers beyond
 // 96 alone cost these kernels a wave): 8, or 7 at 8 columns; masked 8-column
@@ -129,8 +129,8 @@ constexpr int kBatchRI = 1;  // one row per lane group: 42-56 VGPRs, full occupa
 constexpr int batch_wide_waves(int K, bool MASK, int esize) { return K < 8 ? 8 : MASK && esize == 4 ? 1 : 7; }
 
 template <int K, typename XT, bool MASK>
-__global__ __launch_bounds__(kWG, batch_wide_waves(K, MASK, (int)sizeof(XT))) void batch_wide_kernel(const WorkDesc *__restrict__ D, const int *__restrict__ UB,
-                                                         int nd, int total)
+__global__ __launch_bounds__(kWG, batch_wide_waves(K, MASK, (int)sizeof(XT))) void batch_wide_kernel(
+    const WorkDesc *__restrict__ D, const int *__restrict__ UB, int nd, int total)
 {
     extern __shared__ float smem[];
     __shared__ int s_ub[kBatchLdsDesc];

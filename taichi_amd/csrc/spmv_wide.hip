@@ -17,8 +17,8 @@ constexpr int wide_waves(int K, int RI, bool NEAR)
 }
 
 template <int K, typename XT, int RI, bool LONG, bool MASK, bool NEAR>
-__global__ __launch_bounds__(kWG, wide_waves(K, RI, NEAR)) void spmv_wide_kernel(MatView A, const XT *__restrict__ X, int ldx,
-                                                        float *__restrict__ Y, int ldy, int ntiles, int kv)
+__global__ __launch_bounds__(kWG, wide_waves(K, RI, NEAR)) void spmv_wide_kernel(
+    MatView A, const XT *__restrict__ X, int ldx, float *__restrict__ Y, int ldy, int ntiles, int kv)
 {
     extern __shared__ float smem[];
     int tb, te;
